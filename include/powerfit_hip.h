@@ -1,0 +1,127 @@
+/*
+ * powerfit_hip.h — C-ABI of libpowerfit_hip.so, the MI355X (gfx950) engine for
+ * PowerFit's exhaustive 6-D local-cross-correlation (LCC) search.
+ *
+ * Conventions
+ *   - Every function returns int: 0 = ok, < 0 = error (PF_ERR_*); the message
+ *     of the last error on the calling thread is pf_last_error().
+ *   - Arrays are C-order (z, y, x), x fastest, matching numpy.
+ *   - Host buffers are caller-owned and only read/written during the call.
+ *   - *_dev functions take device pointers and a hipStream_t (as void*,
+ *     NULL = default stream); they enqueue and return without syncing.
+ *   - A pf_ctx owns its device memory and stream; one host thread at a time
+ *     may use a ctx; several ctxs may share a device.
+ */
+#ifndef POWERFIT_HIP_H
+#define POWERFIT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_ABI_VERSION 1
+
+#define PF_OK 0
+#define PF_ERR_ARG (-1)
+#define PF_ERR_HIP (-2)
+#define PF_ERR_STATE (-3)
+#define PF_ERR_MEMORY (-4)
+#define PF_ERR_UNSUPPORTED (-5)
+
+/* Scan modes for pf_scan. */
+#define PF_MODE_PLAIN 0   /* one rotation per launch chain, unfused kernels    */
+#define PF_MODE_BATCHED 1 /* `batch` rotations per launch, same kernels         */
+
+int pf_abi_version(void);
+const char *pf_last_error(void);
+int pf_device_count(int *count);
+
+/* ---- Kernel-level drop-ins ------------------------------------------------ */
+
+/* Replaces _extensions.rotate_grid3d(grid, rotmat, radius, out, nearest)
+ * (src/powerfit_em/_extensions.c:7-196, bound by powerfitter.py:339-347).
+ * Host FP64 buffers; rotates about voxel (0,0,0) inside |v| <= radius,
+ * trilinear (nearest = 0) or nearest (nearest != 0, C round), writing only
+ * in-sphere voxels of out.  Requires 0 <= radius <= min(all dims) - 2. */
+int pf_rotate_grid3d(const double *grid, int gz, int gy, int gx, const double *rotmat,
+                     int radius, double *out, int oz, int oy, int ox, int nearest);
+
+/* Replaces _powerfit.conj_multiply(in1, in2, out) (src/powerfit_em/_powerfit.pyx:45-53;
+ * OpenCL conj_multiply powerfitter.py:568-571): out = conj(in1) * in2, complex64. */
+int pf_conj_multiply_dev(const void *in1, const void *in2, void *out, int64_t n,
+                         void *stream);
+
+/* Replaces _powerfit.calc_lcc(gcc, ave, ave2, mask, lcc) (src/powerfit_em/_powerfit.pyx:56-72):
+ * lcc[i] = gcc[i] / sqrt(ave2[i] - ave[i]^2) where mask[i] != 0 (float32, the
+ * variance formed in FP64); other entries untouched. */
+int pf_calc_lcc_dev(const float *gcc, const float *ave, const float *ave2,
+                    const uint8_t *mask, float *lcc, int64_t n, void *stream);
+
+/* Replaces numpy.fft.rfftn / irfftn(s=shape) as used by CPUCorrelator
+ * (powerfitter.py:306-315, 375-393) and clFFT's grfftn_builder (:605-638):
+ * `batch` volumes (nz, ny, nx) float32 <-> (nz, ny, nx/2+1) complex64.
+ * pf_irfftn_dev is normalised like numpy (1/V) and overwrites its input. */
+int pf_rfftn_dev(const float *in, void *out, int nz, int ny, int nx, int batch, void *stream);
+int pf_irfftn_dev(void *in, float *out, int nz, int ny, int nx, int batch, void *stream);
+
+/* ---- Correlator context (BaseCorrelator / CPUCorrelator / GPUCorrelator
+ *      protocol, powerfitter.py:166-556) ------------------------------------ */
+typedef struct pf_ctx pf_ctx;
+
+/* Allocates the per-target state for a (nz, ny, nx) grid on `device`.
+ * flags: reserved, pass 0. */
+int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx **out);
+int pf_ctx_destroy(pf_ctx *ctx);
+/* Use an external hipStream_t (e.g. torch.cuda.Stream.cuda_stream); NULL
+ * restores the ctx's own stream. */
+int pf_ctx_set_stream(pf_ctx *ctx, void *stream);
+
+/* Target: the already normalised and (optionally) Laplace-filtered map Tf
+ * (BaseCorrelator.__init__ :169-176, CPUCorrelator.__init__ :268-277) and the
+ * lcc mask T > 0.05 max(T) (:178-180).  Computes F[Tf], F[Tf^2] on device and
+ * keeps them resident. */
+int pf_set_target(pf_ctx *ctx, const double *target_filtered, const uint8_t *lcc_mask);
+
+/* Template and mask as the mask setter leaves them (:190-220): template
+ * filtered, masked, normalised and masked again; mask possibly core-weighted;
+ * norm_factor N = count(mask != 0); rmax = min(shape) // 2 (:176). */
+int pf_set_template_mask(pf_ctx *ctx, const double *template_prepared, const double *mask,
+                         double norm_factor, int rmax);
+
+/* Rotation set (rotations setter :226-230): K row-major 3x3 FP64 matrices. */
+int pf_set_rotations(pf_ctx *ctx, const double *rotmats, int64_t count);
+
+/* Best LCC := 0, rotation index := 0 (CPUCorrelator.scan :320-321). */
+int pf_reset_result(pf_ctx *ctx);
+
+/* Scan rotations [first, first + count) into the running best/argmax
+ * (CPUCorrelator.scan :323-333; GPUCorrelator.scan :513-538): strict '>',
+ * ties keep the lowest index, global rotation indices.  Enqueues only. */
+int pf_scan(pf_ctx *ctx, int64_t first, int64_t count, int batch, int mode);
+
+/* Copy out best LCC (float32) and rotation index (int32), shape (nz, ny, nx)
+ * — GPUCorrelator's lcc/rot (:536-537).  Synchronises the ctx stream. */
+int pf_get_result(pf_ctx *ctx, float *best_lcc, int32_t *best_rot);
+int pf_sync(pf_ctx *ctx);
+
+/* Kernel timing of subsequent pf_scan calls with HIP events on the ctx
+ * stream (0 = off).  pf_ctx_kernel_time returns, for kernel class `which`
+ * (PF_K_*), the summed milliseconds and launch count since enabling;
+ * it synchronises the stream. */
+#define PF_K_ROTATE 0
+#define PF_K_R2C_X 1
+#define PF_K_C2C_FWD 2
+#define PF_K_CONJ_MUL 3
+#define PF_K_C2C_INV 4
+#define PF_K_C2R_X 5
+#define PF_K_LCC 6
+#define PF_K_COUNT 7
+int pf_ctx_profile(pf_ctx *ctx, int enable);
+int pf_ctx_kernel_time(pf_ctx *ctx, int which, double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

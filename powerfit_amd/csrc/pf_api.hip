@@ -1,0 +1,488 @@
+// C-ABI of libpowerfit_hip.so (declared in include/powerfit_hip.h).
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/powerfit_hip.h"
+#include "pf_internal.h"
+
+using namespace pf;
+
+struct pf_ctx {
+    int device = 0;
+    int nz = 0, ny = 0, nx = 0;
+    int64_t V = 0, H = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    Fft3D fft{};
+    // resident state
+    float* d_template = nullptr;  // f32 prepared template
+    float* d_mask = nullptr;      // f32 mask weights
+    float2* d_F1 = nullptr;       // F[Tf]
+    float2* d_F2 = nullptr;       // F[Tf^2]
+    uint8_t* d_lccmask = nullptr;
+    float* d_best = nullptr;
+    int32_t* d_idx = nullptr;
+    double* d_rot = nullptr;
+    int64_t nrot = 0;
+    double norm = 0.0;
+    int rmax = -1;
+    bool have_target = false, have_template = false;
+    // per-batch work buffers
+    int cap = 0;
+    float* d_real = nullptr;    // [cap][3][V]
+    float2* d_cplx = nullptr;   // [cap][3][H]
+    // profiling
+    bool profile = false;
+    struct Rec {
+        int which;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double acc_ms[PF_K_COUNT] = {};
+    int64_t acc_n[PF_K_COUNT] = {};
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && hipSetDevice(dev) == hipSuccess) ok = true;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+#define PF_DEVICE_GUARD(ctx)                                              \
+    DeviceGuard guard_((ctx)->device);                                    \
+    if (!guard_.ok) {                                                     \
+        set_error("cannot select device %d", (ctx)->device);              \
+        return kErrHip;                                                   \
+    }
+
+int arg_error(const char* msg) {
+    set_error("%s", msg);
+    return kErrArg;
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return kOk;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        set_error("hipMalloc of %zu bytes failed: %s", sizeof(T) * count, hipGetErrorString(e));
+        return kErrMemory;
+    }
+    return kOk;
+}
+
+template <typename T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+hipEvent_t get_event(pf_ctx* c) {
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Run f() (which enqueues launches on c->stream), timing it when profiling.
+template <typename F>
+int timed(pf_ctx* c, int which, F&& f) {
+    if (!c->profile) return f();
+    hipEvent_t a = get_event(c), b = get_event(c);
+    if (!a || !b) {
+        set_error("hipEventCreate failed");
+        return kErrHip;
+    }
+    PF_HIP_CHECK(hipEventRecord(a, c->stream));
+    int s = f();
+    PF_HIP_CHECK(hipEventRecord(b, c->stream));
+    c->recs.push_back({which, a, b});
+    return s;
+}
+
+int drain_profile(pf_ctx* c) {
+    if (c->recs.empty()) return kOk;
+    PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (auto& r : c->recs) {
+        float ms = 0.f;
+        PF_HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+        c->acc_ms[r.which] += ms;
+        c->acc_n[r.which] += 1;
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->recs.clear();
+    return kOk;
+}
+
+int ensure_capacity(pf_ctx* c, int batch) {
+    if (batch <= c->cap) return kOk;
+    PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    dev_free(c->d_real);
+    dev_free(c->d_cplx);
+    c->cap = 0;
+    PF_CHECK_STATUS(dev_alloc(&c->d_real, (size_t)3 * batch * c->V));
+    int s = dev_alloc(&c->d_cplx, (size_t)3 * batch * c->H);
+    if (s) {
+        dev_free(c->d_real);
+        return s;
+    }
+    c->cap = batch;
+    return kOk;
+}
+
+bool radius_ok(int r, int a, int b, int c) {
+    return r >= 0 && r <= std::min(a, std::min(b, c)) - 2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_abi_version(void) { return PF_ABI_VERSION; }
+
+int pf_device_count(int* count) {
+    if (!count) return arg_error("count is null");
+    PF_HIP_CHECK(hipGetDeviceCount(count));
+    return kOk;
+}
+
+int pf_rotate_grid3d(const double* grid, int gz, int gy, int gx, const double* rotmat,
+                     int radius, double* out, int oz, int oy, int ox, int nearest) {
+    if (!grid || !rotmat || !out) return arg_error("null pointer");
+    if (gz < 1 || gy < 1 || gx < 1 || oz < 1 || oy < 1 || ox < 1)
+        return arg_error("invalid grid shape");
+    if (!radius_ok(radius, gz, gy, gx) || !radius_ok(radius, oz, oy, ox))
+        return arg_error("radius must satisfy 0 <= radius <= min(shape) - 2");
+    const size_t ng = (size_t)gz * gy * gx, no = (size_t)oz * oy * ox;
+    double *dg = nullptr, *dr = nullptr, *dout = nullptr;
+    int s = dev_alloc(&dg, ng);
+    if (!s) s = dev_alloc(&dr, 9);
+    if (!s) s = dev_alloc(&dout, no);
+    if (!s) {
+        hipError_t e = hipMemcpy(dg, grid, ng * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dr, rotmat, 9 * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dout, out, no * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_error("rotate_grid3d upload: %s", hipGetErrorString(e));
+            s = kErrHip;
+        }
+    }
+    if (!s) s = launch_rotate_f64(dg, gz, gy, gx, dr, radius, dout, oz, oy, ox, nearest, nullptr);
+    if (!s) {
+        hipError_t e = hipMemcpy(out, dout, no * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            set_error("rotate_grid3d download: %s", hipGetErrorString(e));
+            s = kErrHip;
+        }
+    }
+    dev_free(dg);
+    dev_free(dr);
+    dev_free(dout);
+    return s;
+}
+
+int pf_conj_multiply_dev(const void* in1, const void* in2, void* out, int64_t n, void* stream) {
+    if (n < 0) return arg_error("negative length");
+    if (n > 0 && (!in1 || !in2 || !out)) return arg_error("null pointer");
+    return launch_conj_multiply((const float2*)in1, (const float2*)in2, (float2*)out, n,
+                                (hipStream_t)stream);
+}
+
+int pf_calc_lcc_dev(const float* gcc, const float* ave, const float* ave2, const uint8_t* mask,
+                    float* lcc, int64_t n, void* stream) {
+    if (n < 0) return arg_error("negative length");
+    if (n > 0 && (!gcc || !ave || !ave2 || !mask || !lcc)) return arg_error("null pointer");
+    return launch_calc_lcc(gcc, ave, ave2, mask, lcc, n, (hipStream_t)stream);
+}
+
+int pf_rfftn_dev(const float* in, void* out, int nz, int ny, int nx, int batch, void* stream) {
+    if (!in || !out) return arg_error("null pointer");
+    if (batch < 1) return arg_error("batch must be >= 1");
+    Fft3D f;
+    PF_CHECK_STATUS(fft3d_init(&f, nz, ny, nx));
+    return fft3d_r2c(f, in, (float2*)out, batch, (hipStream_t)stream);
+}
+
+int pf_irfftn_dev(void* in, float* out, int nz, int ny, int nx, int batch, void* stream) {
+    if (!in || !out) return arg_error("null pointer");
+    if (batch < 1) return arg_error("batch must be >= 1");
+    Fft3D f;
+    PF_CHECK_STATUS(fft3d_init(&f, nz, ny, nx));
+    const float scale = (float)(1.0 / ((double)nz * ny * nx));
+    return fft3d_c2r(f, (float2*)in, out, batch, scale, (hipStream_t)stream);
+}
+
+int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** out) {
+    if (!out) return arg_error("out is null");
+    *out = nullptr;
+    if (flags != 0) return arg_error("flags must be 0");
+    if (nz < 1 || ny < 1 || nx < 1) return arg_error("invalid grid shape");
+    int ndev = 0;
+    PF_HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return arg_error("no such device");
+    pf_ctx* c = new pf_ctx();
+    c->device = device;
+    c->nz = nz;
+    c->ny = ny;
+    c->nx = nx;
+    c->V = (int64_t)nz * ny * nx;
+    c->H = (int64_t)nz * ny * (nx / 2 + 1);
+    DeviceGuard g(device);
+    int s = g.ok ? kOk : kErrHip;
+    if (!s) s = fft3d_init(&c->fft, nz, ny, nx);
+    if (!s && hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        s = kErrHip;
+    }
+    c->stream = c->own_stream;
+    if (!s) s = dev_alloc(&c->d_template, c->V);
+    if (!s) s = dev_alloc(&c->d_mask, c->V);
+    if (!s) s = dev_alloc(&c->d_F1, c->H);
+    if (!s) s = dev_alloc(&c->d_F2, c->H);
+    if (!s) s = dev_alloc(&c->d_lccmask, c->V);
+    if (!s) s = dev_alloc(&c->d_best, c->V);
+    if (!s) s = dev_alloc(&c->d_idx, c->V);
+    if (s) {
+        pf_ctx_destroy(c);
+        return s;
+    }
+    *out = c;
+    return pf_reset_result(c);
+}
+
+int pf_ctx_destroy(pf_ctx* c) {
+    if (!c) return kOk;
+    {
+        DeviceGuard g(c->device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        for (auto& r : c->recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : c->pool) (void)hipEventDestroy(e);
+        dev_free(c->d_template);
+        dev_free(c->d_mask);
+        dev_free(c->d_F1);
+        dev_free(c->d_F2);
+        dev_free(c->d_lccmask);
+        dev_free(c->d_best);
+        dev_free(c->d_idx);
+        dev_free(c->d_rot);
+        dev_free(c->d_real);
+        dev_free(c->d_cplx);
+        if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    }
+    delete c;
+    return kOk;
+}
+
+int pf_ctx_set_stream(pf_ctx* c, void* stream) {
+    if (!c) return arg_error("ctx is null");
+    PF_DEVICE_GUARD(c);
+    PF_CHECK_STATUS(drain_profile(c));
+    PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return kOk;
+}
+
+int pf_set_target(pf_ctx* c, const double* target, const uint8_t* lcc_mask) {
+    if (!c || !target || !lcc_mask) return arg_error("null pointer");
+    PF_DEVICE_GUARD(c);
+    double* dt = nullptr;
+    float *t32 = nullptr, *t2 = nullptr;
+    int s = dev_alloc(&dt, c->V);
+    if (!s) s = dev_alloc(&t32, c->V);
+    if (!s) s = dev_alloc(&t2, c->V);
+    if (!s) {
+        hipError_t e = hipMemcpyAsync(dt, target, sizeof(double) * c->V, hipMemcpyHostToDevice,
+                                      c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->d_lccmask, lcc_mask, c->V, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) {
+            set_error("set_target upload: %s", hipGetErrorString(e));
+            s = kErrHip;
+        }
+    }
+    if (!s) s = launch_convert(dt, t32, t2, c->V, c->stream);
+    if (!s) s = fft3d_r2c(c->fft, t32, c->d_F1, 1, c->stream);
+    if (!s) s = fft3d_r2c(c->fft, t2, c->d_F2, 1, c->stream);
+    if (!s && hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_error("set_target: stream sync failed");
+        s = kErrHip;
+    }
+    dev_free(dt);
+    dev_free(t32);
+    dev_free(t2);
+    if (!s) c->have_target = true;
+    return s;
+}
+
+int pf_set_template_mask(pf_ctx* c, const double* tmpl, const double* mask, double norm_factor,
+                         int rmax) {
+    if (!c || !tmpl || !mask) return arg_error("null pointer");
+    if (!(norm_factor > 0)) return arg_error("norm_factor must be > 0 (zero-filled mask)");
+    if (!radius_ok(rmax, c->nz, c->ny, c->nx))
+        return arg_error("rmax must satisfy 0 <= rmax <= min(shape) - 2");
+    PF_DEVICE_GUARD(c);
+    double* dt = nullptr;
+    int s = dev_alloc(&dt, c->V);
+    if (!s && hipMemcpyAsync(dt, tmpl, sizeof(double) * c->V, hipMemcpyHostToDevice,
+                             c->stream) != hipSuccess) {
+        set_error("set_template_mask upload failed");
+        s = kErrHip;
+    }
+    if (!s) s = launch_convert(dt, c->d_template, nullptr, c->V, c->stream);
+    if (!s && hipMemcpyAsync(dt, mask, sizeof(double) * c->V, hipMemcpyHostToDevice,
+                             c->stream) != hipSuccess) {
+        set_error("set_template_mask upload failed");
+        s = kErrHip;
+    }
+    if (!s) s = launch_convert(dt, c->d_mask, nullptr, c->V, c->stream);
+    if (!s && hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_error("set_template_mask: stream sync failed");
+        s = kErrHip;
+    }
+    dev_free(dt);
+    if (!s) {
+        c->norm = norm_factor;
+        c->rmax = rmax;
+        c->have_template = true;
+    }
+    return s;
+}
+
+int pf_set_rotations(pf_ctx* c, const double* rot, int64_t count) {
+    if (!c || (!rot && count > 0)) return arg_error("null pointer");
+    if (count < 0 || count > 0x7fffffffLL) return arg_error("invalid rotation count");
+    PF_DEVICE_GUARD(c);
+    PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    dev_free(c->d_rot);
+    c->nrot = 0;
+    if (count == 0) return kOk;
+    PF_CHECK_STATUS(dev_alloc(&c->d_rot, (size_t)count * 9));
+    PF_HIP_CHECK(hipMemcpy(c->d_rot, rot, sizeof(double) * 9 * count, hipMemcpyHostToDevice));
+    c->nrot = count;
+    return kOk;
+}
+
+int pf_reset_result(pf_ctx* c) {
+    if (!c) return arg_error("ctx is null");
+    PF_DEVICE_GUARD(c);
+    PF_HIP_CHECK(hipMemsetAsync(c->d_best, 0, sizeof(float) * c->V, c->stream));
+    PF_HIP_CHECK(hipMemsetAsync(c->d_idx, 0, sizeof(int32_t) * c->V, c->stream));
+    return kOk;
+}
+
+int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
+    if (!c) return arg_error("ctx is null");
+    if (!c->have_target || !c->have_template || c->nrot == 0) {
+        set_error("First set the template, mask, and rotations.");
+        return kErrState;
+    }
+    if (first < 0 || count < 0 || first + count > c->nrot)
+        return arg_error("rotation range outside the rotation set");
+    if (mode != PF_MODE_PLAIN && mode != PF_MODE_BATCHED) return arg_error("unknown mode");
+    if (mode == PF_MODE_PLAIN) batch = 1;
+    if (batch < 1 || batch > 4096) return arg_error("batch must be in [1, 4096]");
+    PF_DEVICE_GUARD(c);
+    PF_CHECK_STATUS(ensure_capacity(c, batch));
+    const double invV = 1.0 / (double)c->V;
+    for (int64_t k0 = first; k0 < first + count; k0 += batch) {
+        const int nb = (int)std::min<int64_t>(batch, first + count - k0);
+        PF_CHECK_STATUS(timed(c, PF_K_ROTATE, [&] {
+            return launch_rotate_pw(c->d_template, c->d_mask, c->d_rot + 9 * k0, nb, c->nz,
+                                    c->ny, c->nx, c->rmax, c->d_real, c->stream);
+        }));
+        PF_CHECK_STATUS(timed(c, PF_K_R2C_X, [&] {
+            return launch_r2c_x(c->fft, c->d_real, c->d_cplx, (int64_t)3 * nb * c->nz * c->ny,
+                                c->stream);
+        }));
+        const int64_t plane = (int64_t)c->ny * c->fft.nxh;
+        PF_CHECK_STATUS(timed(c, PF_K_C2C_FWD, [&] {
+            int s = launch_c2c_axis(c->fft.py, c->fft.twy, c->d_cplx, c->fft.nxh, c->fft.nxh,
+                                    (int64_t)3 * nb * c->nz, plane, -1, c->stream);
+            if (s) return s;
+            return launch_c2c_axis(c->fft.pz, c->fft.twz, c->d_cplx, plane, plane,
+                                   (int64_t)3 * nb, plane * c->nz, -1, c->stream);
+        }));
+        PF_CHECK_STATUS(timed(c, PF_K_CONJ_MUL, [&] {
+            return launch_conj_mul_pw(c->d_cplx, c->d_F1, c->d_F2, c->H, nb, c->stream);
+        }));
+        PF_CHECK_STATUS(timed(c, PF_K_C2C_INV, [&] {
+            int s = launch_c2c_axis(c->fft.pz, c->fft.twz, c->d_cplx, plane, plane,
+                                    (int64_t)3 * nb, plane * c->nz, 1, c->stream);
+            if (s) return s;
+            return launch_c2c_axis(c->fft.py, c->fft.twy, c->d_cplx, c->fft.nxh, c->fft.nxh,
+                                   (int64_t)3 * nb * c->nz, plane, 1, c->stream);
+        }));
+        PF_CHECK_STATUS(timed(c, PF_K_C2R_X, [&] {
+            return launch_c2r_x(c->fft, c->d_cplx, c->d_real, (int64_t)3 * nb * c->nz * c->ny,
+                                1.0f, c->stream);
+        }));
+        PF_CHECK_STATUS(timed(c, PF_K_LCC, [&] {
+            return launch_lcc_take_best(c->d_real, c->d_lccmask, c->d_best, c->d_idx, c->V, nb,
+                                        k0, c->norm, invV, c->stream);
+        }));
+    }
+    return kOk;
+}
+
+int pf_get_result(pf_ctx* c, float* best, int32_t* idx) {
+    if (!c || !best || !idx) return arg_error("null pointer");
+    PF_DEVICE_GUARD(c);
+    PF_HIP_CHECK(hipMemcpyAsync(best, c->d_best, sizeof(float) * c->V, hipMemcpyDeviceToHost,
+                                c->stream));
+    PF_HIP_CHECK(hipMemcpyAsync(idx, c->d_idx, sizeof(int32_t) * c->V, hipMemcpyDeviceToHost,
+                                c->stream));
+    PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return kOk;
+}
+
+int pf_sync(pf_ctx* c) {
+    if (!c) return arg_error("ctx is null");
+    PF_DEVICE_GUARD(c);
+    PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return kOk;
+}
+
+int pf_ctx_profile(pf_ctx* c, int enable) {
+    if (!c) return arg_error("ctx is null");
+    PF_DEVICE_GUARD(c);
+    PF_CHECK_STATUS(drain_profile(c));
+    c->profile = enable != 0;
+    for (int i = 0; i < PF_K_COUNT; ++i) {
+        c->acc_ms[i] = 0.0;
+        c->acc_n[i] = 0;
+    }
+    return kOk;
+}
+
+int pf_ctx_kernel_time(pf_ctx* c, int which, double* ms, int64_t* launches) {
+    if (!c || !ms || !launches) return arg_error("null pointer");
+    if (which < 0 || which >= PF_K_COUNT) return arg_error("unknown kernel class");
+    PF_DEVICE_GUARD(c);
+    PF_CHECK_STATUS(drain_profile(c));
+    *ms = c->acc_ms[which];
+    *launches = c->acc_n[which];
+    return kOk;
+}
+
+}  // extern "C"

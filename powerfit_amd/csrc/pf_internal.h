@@ -1,0 +1,90 @@
+// Internal declarations shared by the kernel and C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pf_fft.h"
+
+namespace pf {
+
+// Error reporting: the C-ABI returns negative codes and keeps a thread-local
+// message for pf_last_error().
+enum Status : int {
+    kOk = 0,
+    kErrArg = -1,       // invalid argument (shape, size, null pointer, order)
+    kErrHip = -2,       // HIP runtime error
+    kErrState = -3,     // call out of order (e.g. scan before set_target)
+    kErrMemory = -4,    // device allocation failed or request over budget
+    kErrUnsupported = -5,
+};
+
+void set_error(const char* fmt, ...);
+
+#define PF_HIP_CHECK(expr)                                                      \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            ::pf::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                \
+            return ::pf::kErrHip;                                               \
+        }                                                                       \
+    } while (0)
+
+#define PF_CHECK_STATUS(expr)              \
+    do {                                   \
+        int s_ = (expr);                   \
+        if (s_ != 0) return s_;            \
+    } while (0)
+
+// Twiddle table exp(-2 pi i m / n), m < n, for the current device (cached).
+int twiddles(int n, const float2** out);
+
+// 3-D real <-> half-complex transforms over a batch of volumes, layout
+// [batch][nz][ny][nx] real and [batch][nz][ny][nx/2+1] complex (numpy's
+// rfftn layout).  The inverse is unnormalised (multiply by 1/V to get
+// numpy's irfftn) and overwrites its complex input.
+struct Fft3D {
+    int nz, ny, nx, nxh;
+    Plan1D px, py, pz;         // px: length nx/2 (even nx) or nx (odd nx)
+    const float2 *twx, *twy, *twz, *twfull;  // twfull: length nx (R2C post-processing)
+};
+int fft3d_init(Fft3D* f, int nz, int ny, int nx);
+int fft3d_r2c(const Fft3D& f, const float* in, float2* out, int batch, hipStream_t s);
+int fft3d_c2r(const Fft3D& f, float2* in, float* out, int batch, float scale, hipStream_t s);
+
+// Individual launches (also used by the batched search pipeline).
+int launch_r2c_x(const Fft3D& f, const float* in, float2* out, int64_t rows, hipStream_t s);
+int launch_c2r_x(const Fft3D& f, const float2* in, float* out, int64_t rows, float scale,
+                 hipStream_t s);
+int launch_c2c_axis(const Plan1D& p, const float2* tw, float2* data, int64_t stride,
+                    int64_t inner, int64_t outer, int64_t outer_stride, int dir,
+                    hipStream_t s);
+
+// Rotation of template (trilinear) and mask (nearest) for `nrot` rotations
+// starting at `rots` (row-major 3x3 doubles each).  Writes P, W, W^2 planes
+// of batch b at out + (3 b + {0,1,2}) V.  Voxels outside the sphere are 0.
+int launch_rotate_pw(const float* tmpl, const float* mask, const double* rots, int nrot,
+                     int nz, int ny, int nx, int radius, float* out, hipStream_t s);
+// Drop-in rotate_grid3d on FP64 device buffers: writes only in-sphere voxels.
+int launch_rotate_f64(const double* grid, int gz, int gy, int gx, const double* rot,
+                      int radius, double* out, int oz, int oy, int ox, int nearest,
+                      hipStream_t s);
+
+// X[b][g] = conj(X[b][g]) * F_g with F_0 = F_1 = F1, F_2 = F2 (in place).
+int launch_conj_mul_pw(float2* X, const float2* F1, const float2* F2, int64_t H, int nrot,
+                       hipStream_t s);
+int launch_conj_multiply(const float2* a, const float2* b, float2* out, int64_t n,
+                         hipStream_t s);
+
+// Fused calc_lcc + take-best over `nrot` rotations whose unnormalised
+// correlations sit at real + (3 b + {0,1,2}) V.
+int launch_lcc_take_best(const float* real, const uint8_t* mask, float* best, int32_t* idx,
+                         int64_t V, int nrot, int64_t first, double norm, double invV,
+                         hipStream_t s);
+int launch_calc_lcc(const float* gcc, const float* ave, const float* ave2,
+                    const uint8_t* mask, float* lcc, int64_t n, hipStream_t s);
+
+// f64 -> f32 conversion, optionally also writing the square (computed in f64).
+int launch_convert(const double* in, float* out, float* out_sq, int64_t n, hipStream_t s);
+
+}  // namespace pf

@@ -1,0 +1,152 @@
+"""GPU end-to-end parity of the LCC search against the FP64 oracle.
+
+Bar: best LCC within 1e-4 absolute of the oracle everywhere; rotation index
+identical wherever the oracle's winner beats its runner-up by more than 1e-4.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import correlator as oc
+from powerfit_amd import HipCorrelator, PowerFitter, Volume
+from powerfit_amd.rotations import rotations_for_angle
+from powerfit_amd.synth import WORKLOADS, Workload, make_case
+
+from parity import LCC_TOL, assert_scan_parity
+
+pytestmark = pytest.mark.gpu
+NPROC = min(8, os.cpu_count() or 1)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+SMALL = {
+    "t32": Workload("t32", 32, 2.5, 300, 7.0, 20.0, seed=7000),
+    "t32_lap_core": Workload("t32_lap_core", 32, 2.5, 300, 7.0, 20.0, laplace=True,
+                             core_weighted=True, seed=7100),
+    "t30_lap": Workload("t30_lap", 30, 2.5, 250, 7.0, 20.0, laplace=True, seed=7200),
+    "t_odd": Workload("t_odd", 0, 2.5, 200, 7.0, 20.0, core_weighted=True, seed=7300,
+                      shape=(27, 25, 30)),
+}
+
+_cache = {}
+
+
+def case_and_oracle(name, nrot=None, angle=None):
+    key = (name, nrot, angle)
+    if key not in _cache:
+        w = SMALL.get(name) or WORKLOADS[name]
+        c = make_case(w)
+        R = rotations_for_angle(angle or w.angle)
+        if nrot is not None:
+            R = R[:nrot]
+        ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
+                               laplace=c.laplace, nproc=NPROC)
+        _cache[key] = (c, R, ref)
+    return _cache[key]
+
+
+def run_hip(c, R, mode="batched", batch=16):
+    corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode=mode, batch=batch)
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    corr.scan()
+    return corr.lcc, corr.rot
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_small_full_rotation_set(name):
+    c, R, ref = case_and_oracle(name)
+    say(f"scan {name} shape={c.target.shape} K={len(R)} batched(16)")
+    best, rot = run_hip(c, R, "batched", 16)
+    err, dec = assert_scan_parity(best, rot, ref)
+    say(f"  max err {err:.2e}, decisive voxels {dec}")
+
+
+def test_plain_and_batch_sizes_identical():
+    """Batching changes launch shapes only: results are bit-identical."""
+    c, R, ref = case_and_oracle("t32")
+    say("scan t32 plain vs batched(7) vs batched(16)")
+    b16 = run_hip(c, R, "batched", 16)
+    b7 = run_hip(c, R, "batched", 7)
+    p1 = run_hip(c, R, "plain")
+    for other in (b7, p1):
+        assert np.array_equal(b16[0], other[0]) and np.array_equal(b16[1], other[1])
+    assert_scan_parity(*p1, ref)
+
+
+def test_split_ranges_accumulate():
+    """Two scans over [0, K/2) and [K/2, K) without reset equal one scan."""
+    c, R, ref = case_and_oracle("t32")
+    say("scan t32 split ranges")
+    corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, batch=32)
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    corr.scan()
+    whole = (corr.lcc.copy(), corr.rot.copy())
+    corr.reset()
+    h = len(R) // 2
+    corr.enqueue_scan(0, h)
+    corr.enqueue_scan(h, len(R) - h)
+    corr.sync()
+    import ctypes
+    from powerfit_amd import _lib
+    best = np.zeros(c.target.shape, np.float32)
+    rot = np.zeros(c.target.shape, np.int32)
+    _lib.check(_lib.load().pf_get_result(corr._ctx, _lib.ptr(best, _lib.FP),
+                                         _lib.ptr(rot, _lib.I32P)))
+    assert np.array_equal(best, whole[0]) and np.array_equal(rot, whole[1])
+
+
+@pytest.mark.parametrize("name,nrot", [("cpu64", 96), ("lc96", 24), ("search128", 12)])
+def test_configs_rotation_subset(name, nrot):
+    c, R, ref = case_and_oracle(name, nrot=nrot)
+    say(f"scan {name} shape={c.target.shape} first {nrot} rotations")
+    best, rot = run_hip(c, R, "batched", 8)
+    err, dec = assert_scan_parity(best, rot, ref)
+    say(f"  max err {err:.2e}, decisive voxels {dec}")
+
+
+def test_perfect_fit():
+    """Target = template, identity first: LCC 1 at voxel 0 with rotation 0."""
+    w = Workload("pf", 32, 2.5, 300, 7.0, 20.0, seed=7400)
+    c = make_case(w)
+    target = c.template.copy()
+    R = rotations_for_angle(20.0)[:5]
+    corr = HipCorrelator(target)
+    assert corr._lcc_mask[0, 0, 0] == 1
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    corr.scan()
+    assert abs(float(corr.lcc[0, 0, 0]) - 1.0) < 1e-5
+    assert int(corr.rot[0, 0, 0]) == 0
+    assert corr.lcc.max() <= 1.0 + 1e-5
+
+
+def test_powerfitter_interface(tmp_path):
+    c, R, ref = case_and_oracle("t30_lap")
+    say("PowerFitter interface")
+    pf = PowerFitter(Volume(c.target.astype(np.float64)), laplace=c.laplace)
+    pf._rotations = R
+    pf._template = Volume(c.template)
+    pf._mask = Volume(c.mask)
+    pf.directory = str(tmp_path)
+    pf.scan()
+    assert pf._lcc.shape == c.target.shape and pf._lcc.dtype == np.float32
+    assert pf._rot.dtype == np.int32
+    assert_scan_parity(pf._lcc, pf._rot, ref)
+
+
+def test_scan_before_setup_raises():
+    c = make_case(SMALL["t32"])
+    corr = HipCorrelator(c.target.astype(np.float64))
+    with pytest.raises(ValueError):
+        corr.scan()

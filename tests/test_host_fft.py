@@ -1,0 +1,40 @@
+"""The device FFT butterflies, built for the host and run sequentially, match numpy."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "powerfit_amd",
+                   "libpf_fft_host.so")
+FP = ctypes.POINTER(ctypes.c_float)
+
+
+@pytest.fixture(scope="module")
+def host():
+    if not os.path.exists(LIB):
+        pytest.fail("libpf_fft_host.so missing: run python -m powerfit_amd.build")
+    return ctypes.CDLL(LIB)
+
+
+SIZES = list(range(1, 65)) + [72, 75, 81, 96, 98, 120, 125, 128, 135, 143, 160, 192, 256, 343]
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_stockham_matches_numpy(host, n):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    for d in (-1, 1):
+        out = np.zeros(n, np.complex64)
+        assert host.pf_host_fft(n, d, x.ctypes.data_as(FP), out.ctypes.data_as(FP)) >= 0
+        ref = np.fft.fft(x.astype(np.complex128)) if d < 0 else np.fft.ifft(x) * n
+        assert np.abs(out - ref).max() / np.abs(ref).max() < 1e-6
+
+
+def test_plans(host):
+    buf = (ctypes.c_int * 16)()
+    for n, expect in [(128, [8, 8, 2]), (96, [8, 4, 3]), (192, [8, 8, 3]), (160, [8, 4, 5]),
+                      (48, [8, 2, 3]), (7, [7]), (1, [])]:
+        ns = host.pf_host_plan(n, buf, 16)
+        assert list(buf[:ns]) == expect
+        assert int(np.prod(expect)) == n

@@ -1,0 +1,137 @@
+"""Pins of the oracle's correlator restatement (CPU only)."""
+import itertools
+
+import numpy as np
+import pytest
+from scipy.ndimage import laplace
+
+from oracle import core as oracle_core
+from oracle import correlator as oc
+from powerfit_amd.synth import core_indices
+
+
+def test_laplace_wrap_equals_scipy():
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((7, 8, 9))
+    assert np.allclose(oc.laplace_wrap(a), laplace(a, mode="wrap"), atol=1e-12)
+
+
+def test_prepare_target():
+    rng = np.random.default_rng(1)
+    t = rng.random((8, 9, 10)) * 3
+    T, L, rmax, Tf, F1, F2 = oc.prepare_target(t)
+    assert T.max() == 1 and rmax == 4
+    assert np.array_equal(L, (T > 0.05).astype(np.uint8))
+    assert np.allclose(np.fft.irfftn(F1, s=T.shape, axes=(0, 1, 2)), T)
+    assert np.allclose(np.fft.irfftn(F2, s=T.shape, axes=(0, 1, 2)), T ** 2)
+
+
+def test_normalize_template():
+    """tests/test_powerfitter.py:128-133: mean 0, std 1 over the mask."""
+    rng = np.random.default_rng(2)
+    P, W, N = oc.prepare_template(rng.random((8, 9, 10)), np.ones((8, 9, 10)))
+    assert N == 720
+    assert abs(P.mean()) < 1e-12 and abs(P.std() - 1) < 1e-12
+    m = (rng.random((8, 9, 10)) > 0.5).astype(float)
+    P, W, N = oc.prepare_template(rng.random((8, 9, 10)), m)
+    ind = m != 0
+    assert abs(P[ind].mean()) < 1e-12 and abs(P[ind].std() - 1) < 1e-12
+    assert np.all(P[~ind] == 0)
+    with pytest.raises(ValueError):
+        oc.prepare_template(rng.random((4, 4, 4)), np.zeros((4, 4, 4)))
+
+
+def test_perfect_fit_gives_unity_at_origin():
+    """tests/test_powerfitter.py:67-79: template = target, all-ones masks."""
+    rng = np.random.default_rng(3)
+    target = rng.random((5, 6, 7))
+    T, L, rmax, Tf, F1, F2 = oc.prepare_target(target)
+    P, W, N = oc.prepare_template(T, np.ones(T.shape))
+    lcc = oc.lcc_from_rotated(P, W, N, F1, F2, np.ones(T.shape, np.uint8))
+    assert abs(lcc.max() - 1) < 1e-12
+    assert lcc.argmax() == 0
+
+
+def direct_lcc(P, W, N, Tf, lcc_mask):
+    """Brute force: S(t) = sum_x P(x)Tf(x+t) / sqrt(N sum W^2 Tf^2(x+t) - (sum W Tf(x+t))^2)."""
+    out = np.zeros(Tf.shape)
+    for t in itertools.product(*[range(n) for n in Tf.shape]):
+        if not lcc_mask[t]:
+            continue
+        Ts = np.roll(Tf, shift=[-s for s in t], axis=(0, 1, 2))   # Ts[x] = Tf[x + t]
+        g = (P * Ts).sum()
+        a = (W * Ts).sum()
+        b = (W * W * Ts * Ts).sum() * N
+        out[t] = g / np.sqrt(b - a * a)
+    return out
+
+
+@pytest.mark.parametrize("laplace_on,core", [(False, False), (True, False), (False, True),
+                                             (True, True)])
+def test_fft_lcc_equals_direct_space(laplace_on, core):
+    rng = np.random.default_rng(4)
+    shape = (5, 6, 4)
+    target = rng.random(shape)
+    T, L, rmax, Tf, F1, F2 = oc.prepare_target(target, laplace_on)
+    mask = (rng.random(shape) > 0.4).astype(float)
+    if core:
+        mask *= rng.integers(1, 4, shape)
+    P, W, N = oc.prepare_template(rng.random(shape), mask, laplace_on)
+    # use a rotated-like asymmetric placement: shift template and mask
+    Pr, Wr = np.roll(P, (1, 2, 1), (0, 1, 2)), np.roll(W, (1, 2, 1), (0, 1, 2))
+    fast = oc.lcc_from_rotated(Pr, Wr, N, F1, F2, L)
+    slow = direct_lcc(Pr, Wr, N, Tf, L)
+    assert np.allclose(fast, slow, atol=1e-10)
+
+
+def test_scan_take_best_and_runner_up():
+    rng = np.random.default_rng(5)
+    shape = (8, 8, 9)
+    target = rng.random(shape)
+    template = rng.random(shape)
+    mask = (rng.random(shape) > 0.3).astype(float)
+    R = np.stack([np.eye(3)] + [np.linalg.qr(rng.standard_normal((3, 3)))[0] for _ in range(6)])
+    R[1:] *= np.sign(np.linalg.det(R[1:]))[:, None, None]
+    res = oc.scan(target, template, mask, R)
+    T, L, rmax, Tf, F1, F2 = oc.prepare_target(target)
+    P, W, N = oc.prepare_template(template, mask)
+    maps = np.stack([np.zeros(shape)] + [oc.lcc_for_rotation(P, W, N, Rk, rmax, F1, F2, L)
+                                         for Rk in R])
+    # candidate 0 is the initial state (0, index 0); argmax keeps the first maximum
+    k = maps.argmax(axis=0)
+    assert np.array_equal(res.best, maps.max(axis=0))
+    assert np.array_equal(res.rot, np.maximum(k - 1, 0).astype(float))
+    srt = np.sort(maps, axis=0)
+    assert np.array_equal(res.second, srt[-2])
+
+
+def test_parallel_chunks_and_merge_equal_single_scan():
+    rng = np.random.default_rng(6)
+    shape = (8, 9, 8)
+    target = rng.random(shape)
+    template = rng.random(shape)
+    mask = (rng.random(shape) > 0.3).astype(float)
+    R = np.stack([np.linalg.qr(rng.standard_normal((3, 3)))[0] for _ in range(10)])
+    R *= np.sign(np.linalg.det(R))[:, None, None]
+    assert oc.chunks(10, 3) == [(0, 3), (3, 6), (6, 10)]
+    one = oc.scan(target, template, mask, R)
+    par = oc.parallel_scan(target, template, mask, R, nproc=3)
+    assert np.array_equal(one.best, par.best)
+    assert np.array_equal(one.rot, par.rot)
+    assert np.array_equal(one.second, par.second)
+
+
+def test_core_indices_closed_form():
+    """Erosion count (helpers.py:26-34, scipy loop) = taxicab distance to background."""
+    rng = np.random.default_rng(7)
+    for shape in [(6, 7, 5), (9, 9, 9)]:
+        m = (rng.random(shape) > 0.25).astype(float)
+        m[2:5, 2:5, 1:4] = 1
+        ref = core_indices(m)
+        assert np.array_equal(oracle_core.core_indices(m), ref)
+        assert np.array_equal(oracle_core.core_indices_bruteforce(m), ref)
+    box = np.ones((7, 9, 11))
+    d = core_indices(box)
+    zz, yy, xx = np.indices(box.shape)
+    expect = np.minimum.reduce([zz + 1, 7 - zz, yy + 1, 9 - yy, xx + 1, 11 - xx])
+    assert np.array_equal(d, expect)
