@@ -194,3 +194,49 @@ def parallel_scan(target, template, mask, rotations, laplace=False, nproc=None):
         with ctx.Pool(nproc) as pool:
             parts = pool.map(_worker, jobs)
     return merge(parts)
+
+
+def _timed_worker(args, barrier, queue):
+    target, template, mask, rotations, laplace, offset = args
+    T, lcc_mask, rmax, Tf, F1, F2 = prepare_target(target, laplace)
+    P, W, N = prepare_template(template, mask, laplace)
+    best = np.zeros(T.shape)
+    rot = np.zeros(T.shape)
+    barrier.wait()
+    import time
+    t0 = time.perf_counter()
+    for n in range(rotations.shape[0]):
+        s = lcc_for_rotation(P, W, N, rotations[n], rmax, F1, F2, lcc_mask)
+        up = s > best
+        best[up] = s[up]
+        rot[up] = n + offset
+    t1 = time.perf_counter()
+    queue.put((t0, t1, rotations.shape[0]))
+
+
+def timed_parallel_scan(target, template, mask, rotations, laplace=False, nproc=None):
+    """Wall time of the rotation loop of ``PowerFitter._cpu_scan`` (setup excluded).
+
+    ``nproc`` forked workers take the reference's contiguous chunks, build
+    their spectra, meet at a barrier, then scan; returns (rotations, seconds)
+    from the first loop start to the last loop end.
+    """
+    rotations = np.asarray(rotations, dtype=np.float64).reshape(-1, 3, 3)
+    nproc = max(1, min(nproc or os.cpu_count(), rotations.shape[0]))
+    ctx = mp.get_context("fork")
+    barrier = ctx.Barrier(nproc)
+    queue = ctx.Queue()
+    procs = []
+    for lo, hi in chunks(rotations.shape[0], nproc):
+        args = (target, template, mask, rotations[lo:hi], laplace, lo)
+        p = ctx.Process(target=_timed_worker, args=(args, barrier, queue))
+        p.start()
+        procs.append(p)
+    res = [queue.get() for _ in procs]
+    for p in procs:
+        p.join()
+        if p.exitcode != 0:
+            raise RuntimeError(f"oracle worker failed with exit code {p.exitcode}")
+    t0 = min(r[0] for r in res)
+    t1 = max(r[1] for r in res)
+    return sum(r[2] for r in res), t1 - t0
