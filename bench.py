@@ -30,9 +30,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def alg_bytes_per_launch(kernel, V, H, B):
+def alg_bytes_per_launch(kernel, V, H, B, binary=False):
     """Algorithmic bytes one launch of a kernel class must move (f32 = 4 B,
-    complex64 = 8 B), for B rotations per launch."""
+    complex64 = 8 B), for B rotations per launch.  With a 0/1 mask the fused
+    path transforms W once (FFT(W^2) = FFT(W)): 2 xy-spectra instead of 3."""
+    nsp = 2 if binary else 3
     per_rot = {
         "rotate": 8 * V + 12 * V,          # read template+mask, write P, W, W^2
         "r2c_x": 12 * V + 24 * H,          # read 3 real grids, write 3 half spectra
@@ -41,8 +43,13 @@ def alg_bytes_per_launch(kernel, V, H, B):
         "c2c_inv": 48 * H,
         "c2r_x": 24 * H + 12 * V,
         "lcc": 12 * V,                     # G, A, B
+        # fused kernels (xy-spectra in numpy layout between them)
+        "fwd_plane": 8 * V + 8 * nsp * H,  # gather template+mask, write the xy-spectra
+        "zcorr": 8 * nsp * H + 24 * H,     # xy-spectra in, 3 correlation spectra out
+        "inv_plane": 24 * H,               # 3 spectra in
     }[kernel]
-    per_launch = {"conj_mul": 16 * H, "lcc": V + 16 * V}.get(kernel, 0)  # F1,F2 / mask, best+idx r+w
+    per_launch = {"conj_mul": 16 * H, "lcc": V + 16 * V, "zcorr": 16 * H,
+                  "inv_plane": V + 16 * V}.get(kernel, 0)  # F1,F2 / mask, keys r+w
     return per_rot * B + per_launch
 
 
@@ -78,7 +85,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--mode", default="batched", choices=["batched", "plain"])
+    ap.add_argument("--mode", default="fused", choices=["fused", "batched", "plain", "auto"])
     ap.add_argument("--workload", default="search128")
     ap.add_argument("--plain-rotations", type=int, default=256,
                     help="rotations timed on the plain one-rotation-per-launch path")
@@ -137,7 +144,8 @@ def main(argv=None):
     dom = max(ktimes, key=lambda k: ktimes[k][0])
     dom_ms, dom_n = ktimes[dom]
     avg_s = dom_ms / 1e3 / dom_n
-    achieved = alg_bytes_per_launch(dom, V, H, B) / avg_s / 1e9
+    binary = bool(np.all((case.mask == 0) | (case.mask == 1)))
+    achieved = alg_bytes_per_launch(dom, V, H, B, binary) / avg_s / 1e9
     path_gbs = value * path_bytes_per_rotation(V, H) / 1e9
 
     # plain one-rotation-per-launch path on the same inputs
@@ -189,7 +197,8 @@ def main(argv=None):
         "path_roofline": {"bytes_per_rotation": path_bytes_per_rotation(V, H),
                           "achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": path_gbs / HBM_PEAK_GBS},
-        "kernels_ms_per_rotation": {k: ktimes[k][0] / rot_done for k in ktimes},
+        "kernels_ms_per_rotation": {k: ktimes[k][0] / rot_done for k in ktimes
+                                    if ktimes[k][1]},
         "plain_value": plain_value,
         "speedup_vs_plain": (value / plain_value) if plain_value else None,
         "cpu_baseline": cpu,

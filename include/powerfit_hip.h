@@ -31,8 +31,18 @@ extern "C" {
 #define PF_ERR_UNSUPPORTED (-5)
 
 /* Scan modes for pf_scan. */
-#define PF_MODE_PLAIN 0   /* one rotation per launch chain, unfused kernels    */
-#define PF_MODE_BATCHED 1 /* `batch` rotations per launch, same kernels         */
+#define PF_MODE_PLAIN 0   /* one rotation per launch chain, unfused kernels        */
+#define PF_MODE_BATCHED 1 /* `batch` rotations per launch, same unfused kernels     */
+#define PF_MODE_FUSED 2   /* `batch` rotations per launch through the three fused
+                             kernels (cubic grids with a compiled size only)        */
+#define PF_MODE_AUTO 3    /* PF_MODE_FUSED where available, else PF_MODE_BATCHED   */
+/* Stage-mixing modes for testing: one fused stage, the rest unfused. */
+#define PF_MODE_MIX_FWD 4  /* fused rotate + forward xy                             */
+#define PF_MODE_MIX_Z 5    /* fused z correlate                                     */
+#define PF_MODE_MIX_INV 6  /* fused inverse xy + LCC                                */
+
+/* pf_ctx_create flags */
+#define PF_FLAG_NO_BINARY_MASK 1u /* never use the W^2 = W shortcut for 0/1 masks  */
 
 int pf_abi_version(void);
 const char *pf_last_error(void);
@@ -71,7 +81,7 @@ int pf_irfftn_dev(void *in, float *out, int nz, int ny, int nx, int batch, void 
 typedef struct pf_ctx pf_ctx;
 
 /* Allocates the per-target state for a (nz, ny, nx) grid on `device`.
- * flags: reserved, pass 0. */
+ * flags: PF_FLAG_* bits (0 for defaults). */
 int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx **out);
 int pf_ctx_destroy(pf_ctx *ctx);
 /* Use an external hipStream_t (e.g. torch.cuda.Stream.cuda_stream); NULL
@@ -117,7 +127,12 @@ int pf_sync(pf_ctx *ctx);
 #define PF_K_C2C_INV 4
 #define PF_K_C2R_X 5
 #define PF_K_LCC 6
-#define PF_K_COUNT 7
+#define PF_K_FWD_PLANE 7
+#define PF_K_ZCORR 8
+#define PF_K_INV_PLANE 9
+#define PF_K_COUNT 10
+/* 1 when PF_MODE_FUSED is available for this ctx's grid. */
+int pf_ctx_fused_available(pf_ctx *ctx);
 int pf_ctx_profile(pf_ctx *ctx, int enable);
 int pf_ctx_kernel_time(pf_ctx *ctx, int which, double *ms, int64_t *launches);
 

@@ -12,9 +12,13 @@ LIB_PATH = os.path.join(HERE, "libpowerfit_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "powerfit_hip.h")
 
 ABI_VERSION = 1
-MODE_PLAIN = 0
-MODE_BATCHED = 1
-KERNELS = ["rotate", "r2c_x", "c2c_fwd", "conj_mul", "c2c_inv", "c2r_x", "lcc"]
+MODES = {"plain": 0, "batched": 1, "fused": 2, "auto": 3,
+         "mix_fwd": 4, "mix_z": 5, "mix_inv": 6}
+MODE_PLAIN = MODES["plain"]
+MODE_BATCHED = MODES["batched"]
+FLAG_NO_BINARY_MASK = 1
+KERNELS = ["rotate", "r2c_x", "c2c_fwd", "conj_mul", "c2c_inv", "c2r_x", "lcc",
+           "fwd_plane", "zcorr", "inv_plane"]
 
 _c_int, _c_i64, _c_u32, _c_dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_uint32, ctypes.c_double
 _vp = ctypes.c_void_p
@@ -44,6 +48,7 @@ _SIGNATURES = {
     "pf_get_result": ([_vp, _fp, _i32p], _c_int),
     "pf_sync": ([_vp], _c_int),
     "pf_ctx_profile": ([_vp, _c_int], _c_int),
+    "pf_ctx_fused_available": ([_vp], _c_int),
     "pf_ctx_kernel_time": ([_vp, _c_int, _dp, ctypes.POINTER(_c_i64)], _c_int),
 }
 

@@ -106,13 +106,21 @@ class BaseCorrelator:
 class HipCorrelator(BaseCorrelator):
     """LCC search on one MI355X through libpowerfit_hip.so.
 
-    mode  : "plain" (one rotation per launch chain) or "batched"
-            (``batch`` rotations per launch).
-    batch : rotations per launch in batched mode.
+    mode  : "plain" (one rotation per launch chain, unfused kernels),
+            "batched" (``batch`` rotations per launch, unfused kernels),
+            "fused" (``batch`` rotations per launch through the three fused
+            kernels; cubic grids of a compiled size), or "auto" (fused where
+            available, else batched).  "mix_fwd"/"mix_z"/"mix_inv" run one
+            fused stage with the rest unfused (for testing).
+    batch : rotations per launch.
+    binary_shortcut : use FFT(W^2) = FFT(W) when the mask is 0/1.
     """
 
-    def __init__(self, target, laplace=False, device=0, mode="batched", batch=16):
+    def __init__(self, target, laplace=False, device=0, mode="auto", batch=64,
+                 binary_shortcut=True):
         super().__init__(target, laplace=laplace)
+        if mode not in _lib.MODES:
+            raise ValueError(f"unknown mode {mode!r}")
         self._dev = device
         self.mode = mode
         self.batch = int(batch)
@@ -120,7 +128,9 @@ class HipCorrelator(BaseCorrelator):
         self._libref = lib
         ctx = ctypes.c_void_p()
         nz, ny, nx = self._target.shape
-        _lib.check(lib.pf_ctx_create(device, nz, ny, nx, 0, ctypes.byref(ctx)), "pf_ctx_create")
+        flags = 0 if binary_shortcut else _lib.FLAG_NO_BINARY_MASK
+        _lib.check(lib.pf_ctx_create(device, nz, ny, nx, flags, ctypes.byref(ctx)),
+                   "pf_ctx_create")
         self._ctx = ctx
         tf = self._laplace_filter(self._target) if self._laplace else self._target
         tf = np.ascontiguousarray(tf, dtype=np.float64)
@@ -162,7 +172,7 @@ class HipCorrelator(BaseCorrelator):
         super().scan()
         nrot = self._rotations.shape[0]
         count = nrot - first if count is None else count
-        mode = _lib.MODE_PLAIN if self.mode == "plain" else _lib.MODE_BATCHED
+        mode = _lib.MODES[self.mode]
         lib = self._libref
         _lib.check(lib.pf_reset_result(self._ctx), "pf_reset_result")
         _lib.check(lib.pf_scan(self._ctx, first, count, self.batch, mode), "pf_scan")
@@ -192,8 +202,12 @@ class HipCorrelator(BaseCorrelator):
 
     def enqueue_scan(self, first, count):
         """Enqueue a scan of [first, first+count) without reset or sync."""
-        mode = _lib.MODE_PLAIN if self.mode == "plain" else _lib.MODE_BATCHED
-        _lib.check(self._libref.pf_scan(self._ctx, first, count, self.batch, mode), "pf_scan")
+        _lib.check(self._libref.pf_scan(self._ctx, first, count, self.batch,
+                                        _lib.MODES[self.mode]), "pf_scan")
+
+    @property
+    def fused_available(self):
+        return bool(self._libref.pf_ctx_fused_available(self._ctx))
 
     def reset(self):
         _lib.check(self._libref.pf_reset_result(self._ctx), "pf_reset_result")

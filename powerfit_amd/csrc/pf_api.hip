@@ -6,6 +6,7 @@
 
 #include "../../include/powerfit_hip.h"
 #include "pf_internal.h"
+#include "pf_rotate.h"
 
 using namespace pf;
 
@@ -22,16 +23,18 @@ struct pf_ctx {
     float2* d_F1 = nullptr;       // F[Tf]
     float2* d_F2 = nullptr;       // F[Tf^2]
     uint8_t* d_lccmask = nullptr;
-    float* d_best = nullptr;
-    int32_t* d_idx = nullptr;
+    float2* d_tm = nullptr;       // packed (template, mask) for the fused path
+    unsigned long long* d_keys = nullptr;  // running (LCC, index) keys
+    uint32_t flags = 0;
+    bool mask_binary = false;
     double* d_rot = nullptr;
     int64_t nrot = 0;
     double norm = 0.0;
     int rmax = -1;
     bool have_target = false, have_template = false;
     // per-batch work buffers
-    int cap = 0;
-    float* d_real = nullptr;    // [cap][3][V]
+    int cap_real = 0, cap_cplx = 0;
+    float* d_real = nullptr;    // [cap][3][V]  (unfused stages only)
     float2* d_cplx = nullptr;   // [cap][3][H]
     // profiling
     bool profile = false;
@@ -132,19 +135,21 @@ int drain_profile(pf_ctx* c) {
     return kOk;
 }
 
-int ensure_capacity(pf_ctx* c, int batch) {
-    if (batch <= c->cap) return kOk;
-    PF_HIP_CHECK(hipStreamSynchronize(c->stream));
-    dev_free(c->d_real);
-    dev_free(c->d_cplx);
-    c->cap = 0;
-    PF_CHECK_STATUS(dev_alloc(&c->d_real, (size_t)3 * batch * c->V));
-    int s = dev_alloc(&c->d_cplx, (size_t)3 * batch * c->H);
-    if (s) {
-        dev_free(c->d_real);
-        return s;
+int ensure_capacity(pf_ctx* c, int batch, bool need_real) {
+    if (batch > c->cap_cplx) {
+        PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+        dev_free(c->d_cplx);
+        c->cap_cplx = 0;
+        PF_CHECK_STATUS(dev_alloc(&c->d_cplx, (size_t)3 * batch * c->H));
+        c->cap_cplx = batch;
     }
-    c->cap = batch;
+    if (need_real && batch > c->cap_real) {
+        PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+        dev_free(c->d_real);
+        c->cap_real = 0;
+        PF_CHECK_STATUS(dev_alloc(&c->d_real, (size_t)3 * batch * c->V));
+        c->cap_real = batch;
+    }
     return kOk;
 }
 
@@ -233,13 +238,14 @@ int pf_irfftn_dev(void* in, float* out, int nz, int ny, int nx, int batch, void*
 int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** out) {
     if (!out) return arg_error("out is null");
     *out = nullptr;
-    if (flags != 0) return arg_error("flags must be 0");
+    if (flags & ~PF_FLAG_NO_BINARY_MASK) return arg_error("unknown flags");
     if (nz < 1 || ny < 1 || nx < 1) return arg_error("invalid grid shape");
     int ndev = 0;
     PF_HIP_CHECK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return arg_error("no such device");
     pf_ctx* c = new pf_ctx();
     c->device = device;
+    c->flags = flags;
     c->nz = nz;
     c->ny = ny;
     c->nx = nx;
@@ -258,8 +264,8 @@ int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** o
     if (!s) s = dev_alloc(&c->d_F1, c->H);
     if (!s) s = dev_alloc(&c->d_F2, c->H);
     if (!s) s = dev_alloc(&c->d_lccmask, c->V);
-    if (!s) s = dev_alloc(&c->d_best, c->V);
-    if (!s) s = dev_alloc(&c->d_idx, c->V);
+    if (!s) s = dev_alloc(&c->d_tm, c->V);
+    if (!s) s = dev_alloc(&c->d_keys, c->V);
     if (s) {
         pf_ctx_destroy(c);
         return s;
@@ -283,8 +289,8 @@ int pf_ctx_destroy(pf_ctx* c) {
         dev_free(c->d_F1);
         dev_free(c->d_F2);
         dev_free(c->d_lccmask);
-        dev_free(c->d_best);
-        dev_free(c->d_idx);
+        dev_free(c->d_tm);
+        dev_free(c->d_keys);
         dev_free(c->d_rot);
         dev_free(c->d_real);
         dev_free(c->d_cplx);
@@ -356,12 +362,16 @@ int pf_set_template_mask(pf_ctx* c, const double* tmpl, const double* mask, doub
         s = kErrHip;
     }
     if (!s) s = launch_convert(dt, c->d_mask, nullptr, c->V, c->stream);
+    if (!s) s = launch_pack_tm(c->d_template, c->d_mask, c->d_tm, c->V, c->stream);
     if (!s && hipStreamSynchronize(c->stream) != hipSuccess) {
         set_error("set_template_mask: stream sync failed");
         s = kErrHip;
     }
     dev_free(dt);
     if (!s) {
+        bool binary = true;
+        for (int64_t i = 0; i < c->V && binary; ++i) binary = (mask[i] == 0.0 || mask[i] == 1.0);
+        c->mask_binary = binary && !(c->flags & PF_FLAG_NO_BINARY_MASK);
         c->norm = norm_factor;
         c->rmax = rmax;
         c->have_template = true;
@@ -386,9 +396,7 @@ int pf_set_rotations(pf_ctx* c, const double* rot, int64_t count) {
 int pf_reset_result(pf_ctx* c) {
     if (!c) return arg_error("ctx is null");
     PF_DEVICE_GUARD(c);
-    PF_HIP_CHECK(hipMemsetAsync(c->d_best, 0, sizeof(float) * c->V, c->stream));
-    PF_HIP_CHECK(hipMemsetAsync(c->d_idx, 0, sizeof(int32_t) * c->V, c->stream));
-    return kOk;
+    return launch_fill_keys(c->d_keys, c->V, lcc_key(0.0f, 0u), c->stream);
 }
 
 int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
@@ -399,60 +407,108 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     }
     if (first < 0 || count < 0 || first + count > c->nrot)
         return arg_error("rotation range outside the rotation set");
-    if (mode != PF_MODE_PLAIN && mode != PF_MODE_BATCHED) return arg_error("unknown mode");
+    if (mode < PF_MODE_PLAIN || mode > PF_MODE_MIX_INV) return arg_error("unknown mode");
+    const bool fused_ok = fused_supported(c->nz, c->ny, c->nx);
+    if (mode == PF_MODE_AUTO) mode = fused_ok ? PF_MODE_FUSED : PF_MODE_BATCHED;
+    if (mode >= PF_MODE_FUSED && !fused_ok) {
+        set_error("fused pipeline not available for grid (%d, %d, %d)", c->nz, c->ny, c->nx);
+        return kErrUnsupported;
+    }
     if (mode == PF_MODE_PLAIN) batch = 1;
     if (batch < 1 || batch > 4096) return arg_error("batch must be in [1, 4096]");
+    // stage A: rotate + forward xy; B: z correlate; C: inverse xy + LCC
+    const bool fa = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_FWD;
+    const bool fb = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_Z;
+    const bool fc = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_INV;
     PF_DEVICE_GUARD(c);
-    PF_CHECK_STATUS(ensure_capacity(c, batch));
+    PF_CHECK_STATUS(ensure_capacity(c, batch, !(fa && fc)));
     const double invV = 1.0 / (double)c->V;
+    const int64_t plane = (int64_t)c->ny * c->fft.nxh;
+    FusedArgs fa_args{};
+    fa_args.n = c->nx;
+    fa_args.tm = c->d_tm;
+    fa_args.rmax = c->rmax;
+    fa_args.work = c->d_cplx;
+    fa_args.F1 = c->d_F1;
+    fa_args.F2 = c->d_F2;
+    fa_args.lcc_mask = c->d_lccmask;
+    fa_args.keys = c->d_keys;
+    fa_args.norm = c->norm;
+    fa_args.invV = invV;
+    fa_args.chunk_z = 8;
+    fa_args.chunk_lcc = 16;
     for (int64_t k0 = first; k0 < first + count; k0 += batch) {
         const int nb = (int)std::min<int64_t>(batch, first + count - k0);
-        PF_CHECK_STATUS(timed(c, PF_K_ROTATE, [&] {
-            return launch_rotate_pw(c->d_template, c->d_mask, c->d_rot + 9 * k0, nb, c->nz,
-                                    c->ny, c->nx, c->rmax, c->d_real, c->stream);
-        }));
-        PF_CHECK_STATUS(timed(c, PF_K_R2C_X, [&] {
-            return launch_r2c_x(c->fft, c->d_real, c->d_cplx, (int64_t)3 * nb * c->nz * c->ny,
-                                c->stream);
-        }));
-        const int64_t plane = (int64_t)c->ny * c->fft.nxh;
-        PF_CHECK_STATUS(timed(c, PF_K_C2C_FWD, [&] {
-            int s = launch_c2c_axis(c->fft.py, c->fft.twy, c->d_cplx, c->fft.nxh, c->fft.nxh,
-                                    (int64_t)3 * nb * c->nz, plane, -1, c->stream);
-            if (s) return s;
-            return launch_c2c_axis(c->fft.pz, c->fft.twz, c->d_cplx, plane, plane,
-                                   (int64_t)3 * nb, plane * c->nz, -1, c->stream);
-        }));
-        PF_CHECK_STATUS(timed(c, PF_K_CONJ_MUL, [&] {
-            return launch_conj_mul_pw(c->d_cplx, c->d_F1, c->d_F2, c->H, nb, c->stream);
-        }));
-        PF_CHECK_STATUS(timed(c, PF_K_C2C_INV, [&] {
-            int s = launch_c2c_axis(c->fft.pz, c->fft.twz, c->d_cplx, plane, plane,
-                                    (int64_t)3 * nb, plane * c->nz, 1, c->stream);
-            if (s) return s;
-            return launch_c2c_axis(c->fft.py, c->fft.twy, c->d_cplx, c->fft.nxh, c->fft.nxh,
-                                   (int64_t)3 * nb * c->nz, plane, 1, c->stream);
-        }));
-        PF_CHECK_STATUS(timed(c, PF_K_C2R_X, [&] {
-            return launch_c2r_x(c->fft, c->d_cplx, c->d_real, (int64_t)3 * nb * c->nz * c->ny,
-                                1.0f, c->stream);
-        }));
-        PF_CHECK_STATUS(timed(c, PF_K_LCC, [&] {
-            return launch_lcc_take_best(c->d_real, c->d_lccmask, c->d_best, c->d_idx, c->V, nb,
-                                        k0, c->norm, invV, c->stream);
-        }));
+        fa_args.rots = c->d_rot + 9 * k0;
+        fa_args.nrot = nb;
+        fa_args.first = k0;
+        if (fa) {
+            // W^2 = W shortcut only when the z stage is fused (it fills grid 2)
+            fa_args.binary_mask = c->mask_binary && fb;
+            PF_CHECK_STATUS(timed(c, PF_K_FWD_PLANE, [&] { return launch_fused(fa_args, 0, c->stream); }));
+        } else {
+            PF_CHECK_STATUS(timed(c, PF_K_ROTATE, [&] {
+                return launch_rotate_pw(c->d_template, c->d_mask, c->d_rot + 9 * k0, nb, c->nz,
+                                        c->ny, c->nx, c->rmax, c->d_real, c->stream);
+            }));
+            PF_CHECK_STATUS(timed(c, PF_K_R2C_X, [&] {
+                return launch_r2c_x(c->fft, c->d_real, c->d_cplx, (int64_t)3 * nb * c->nz * c->ny,
+                                    c->stream);
+            }));
+            PF_CHECK_STATUS(timed(c, PF_K_C2C_FWD, [&] {
+                return launch_c2c_axis(c->fft.py, c->fft.twy, c->d_cplx, c->fft.nxh, c->fft.nxh,
+                                       (int64_t)3 * nb * c->nz, plane, -1, c->stream);
+            }));
+        }
+        if (fb) {
+            fa_args.binary_mask = c->mask_binary;
+            PF_CHECK_STATUS(timed(c, PF_K_ZCORR, [&] { return launch_fused(fa_args, 1, c->stream); }));
+        } else {
+            PF_CHECK_STATUS(timed(c, PF_K_C2C_FWD, [&] {
+                return launch_c2c_axis(c->fft.pz, c->fft.twz, c->d_cplx, plane, plane,
+                                       (int64_t)3 * nb, plane * c->nz, -1, c->stream);
+            }));
+            PF_CHECK_STATUS(timed(c, PF_K_CONJ_MUL, [&] {
+                return launch_conj_mul_pw(c->d_cplx, c->d_F1, c->d_F2, c->H, nb, c->stream);
+            }));
+            PF_CHECK_STATUS(timed(c, PF_K_C2C_INV, [&] {
+                return launch_c2c_axis(c->fft.pz, c->fft.twz, c->d_cplx, plane, plane,
+                                       (int64_t)3 * nb, plane * c->nz, 1, c->stream);
+            }));
+        }
+        if (fc) {
+            PF_CHECK_STATUS(timed(c, PF_K_INV_PLANE, [&] { return launch_fused(fa_args, 2, c->stream); }));
+        } else {
+            PF_CHECK_STATUS(timed(c, PF_K_C2C_INV, [&] {
+                return launch_c2c_axis(c->fft.py, c->fft.twy, c->d_cplx, c->fft.nxh, c->fft.nxh,
+                                       (int64_t)3 * nb * c->nz, plane, 1, c->stream);
+            }));
+            PF_CHECK_STATUS(timed(c, PF_K_C2R_X, [&] {
+                return launch_c2r_x(c->fft, c->d_cplx, c->d_real, (int64_t)3 * nb * c->nz * c->ny,
+                                    1.0f, c->stream);
+            }));
+            PF_CHECK_STATUS(timed(c, PF_K_LCC, [&] {
+                return launch_lcc_take_best(c->d_real, c->d_lccmask, c->d_keys, c->V, nb, k0,
+                                            c->norm, invV, c->stream);
+            }));
+        }
     }
     return kOk;
+}
+
+int pf_ctx_fused_available(pf_ctx* c) {
+    if (!c) return arg_error("ctx is null");
+    return fused_supported(c->nz, c->ny, c->nx) ? 1 : 0;
 }
 
 int pf_get_result(pf_ctx* c, float* best, int32_t* idx) {
     if (!c || !best || !idx) return arg_error("null pointer");
     PF_DEVICE_GUARD(c);
-    PF_HIP_CHECK(hipMemcpyAsync(best, c->d_best, sizeof(float) * c->V, hipMemcpyDeviceToHost,
-                                c->stream));
-    PF_HIP_CHECK(hipMemcpyAsync(idx, c->d_idx, sizeof(int32_t) * c->V, hipMemcpyDeviceToHost,
-                                c->stream));
+    std::vector<unsigned long long> keys(c->V);
+    PF_HIP_CHECK(hipMemcpyAsync(keys.data(), c->d_keys, sizeof(unsigned long long) * c->V,
+                                hipMemcpyDeviceToHost, c->stream));
     PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < c->V; ++i) lcc_unkey(keys[i], best + i, idx + i);
     return kOk;
 }
 

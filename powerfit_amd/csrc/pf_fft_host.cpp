@@ -44,3 +44,43 @@ extern "C" int pf_host_plan(int n, int* radix, int max) {
     for (int s = 0; s < p.nstages && s < max; ++s) radix[s] = p.radix[s];
     return p.nstages;
 }
+
+// Host emulation of the register group FFT (pf_group_fft.h) for the
+// (N, T) pairs the fused kernels use.
+#include "pf_group_fft.h"
+
+template <int N, int T>
+static int group_fft_host(int dir, const float* in, float* out) {
+    constexpr int E = N / T;
+    std::vector<float2> tw(N), ex(N);
+    for (int m = 0; m < N; ++m) {
+        const double ang = -2.0 * M_PI * (double)m / (double)N;
+        tw[m] = pf::c2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    std::vector<float2> regs((size_t)T * E);
+    auto* v = reinterpret_cast<float2(*)[E]>(regs.data());
+    for (int t = 0; t < T; ++t)
+        for (int e = 0; e < E; ++e) v[t][e] = pf::c2(in[2 * (t + T * e)], in[2 * (t + T * e) + 1]);
+    if (dir < 0) pf::GroupStage<N, T, -1, 0>::emulate(v, ex.data(), tw.data());
+    else pf::GroupStage<N, T, 1, 0>::emulate(v, ex.data(), tw.data());
+    for (int t = 0; t < T; ++t)
+        for (int e = 0; e < E; ++e) {
+            out[2 * (t + T * e)] = v[t][e].x;
+            out[2 * (t + T * e) + 1] = v[t][e].y;
+        }
+    return pf::GroupStage<N, T, -1, 0>::P.ns;
+}
+
+#define PF_GROUP_CASE(N, T) \
+    if (n == N && t == T) return group_fft_host<N, T>(dir, in, out);
+
+extern "C" int pf_host_group_fft(int n, int t, int dir, const float* in, float* out) {
+    PF_GROUP_CASE(16, 1) PF_GROUP_CASE(32, 4) PF_GROUP_CASE(32, 2)
+    PF_GROUP_CASE(48, 4) PF_GROUP_CASE(64, 8) PF_GROUP_CASE(64, 4)
+    PF_GROUP_CASE(96, 4) PF_GROUP_CASE(96, 8) PF_GROUP_CASE(128, 8)
+    PF_GROUP_CASE(128, 16) PF_GROUP_CASE(160, 8) PF_GROUP_CASE(192, 8)
+    PF_GROUP_CASE(80, 4) PF_GROUP_CASE(40, 4) PF_GROUP_CASE(56, 4) PF_GROUP_CASE(100, 5)
+    return -1;
+}
+
+extern "C" int pf_host_pick_T(int n, int emax, int tmax) { return pf::pick_T(n, emax, tmax); }

@@ -1,0 +1,209 @@
+// Compile-time register FFTs executed by a group of T threads.
+//
+// An N-point transform is held by T threads, E = N / T elements each, in the
+// strided layout v[e] = x[t + T e] (t = thread index in the group).  Each
+// Stockham stage of radix R (R | E) runs E / R butterflies per thread
+// entirely from registers — butterfly j = t + T b reads elements
+// j + r N / R = t + T (b + r E / R), i.e. v[b + r E / R] — writes its outputs
+// to an LDS exchange buffer at their natural positions, and reads the next
+// stage's inputs back in the strided layout.  The result is again strided,
+// in natural order.  All loops and radices are compile-time, so everything
+// unrolls and register arrays are statically indexed.
+//
+// Every call contains block-wide barriers: all threads of the block must call
+// it the same number of times (pass active = false for idle groups).
+#pragma once
+#include "pf_fft.h"
+
+namespace pf {
+
+struct GPlan {
+    int T = 0, E = 0, ns = 0;
+    int r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Radix plan for N points over T threads: greedy largest radix dividing both
+// the remaining length and E.  ns = -1 when impossible.
+constexpr GPlan gplan_for(int N, int T) {
+    GPlan p;
+    p.T = T;
+    if (T < 1 || N % T != 0) {
+        p.ns = -1;
+        return p;
+    }
+    p.E = N / T;
+    int rem = N;
+    const int pref[7] = {16, 8, 4, 2, 3, 5, 7};
+    while (rem > 1) {
+        int pick = 0;
+        for (int i = 0; i < 7; ++i) {
+            const int R = pref[i];
+            if (rem % R == 0 && p.E % R == 0) {
+                pick = R;
+                break;
+            }
+        }
+        if (pick == 0 || p.ns == 8) {
+            p.ns = -1;
+            return p;
+        }
+        p.r[p.ns++] = pick;
+        rem /= pick;
+    }
+    return p;
+}
+
+// Threads per transform for N: fewest stages with E <= emax, ties -> most threads.
+constexpr int pick_T(int N, int emax, int tmax) {
+    int best_t = 0, best_ns = 100;
+    for (int T = tmax; T >= 1; --T) {
+        if (N % T != 0 || N / T > emax) continue;
+        GPlan p = gplan_for(N, T);
+        if (p.ns < 0) continue;
+        if (p.ns < best_ns) {
+            best_ns = p.ns;
+            best_t = T;
+        }
+    }
+    return best_t;
+}
+
+// 16-point DFT as 4 x 4 with inter-stage twiddles W16^(n2 k1).
+template <int DIR>
+PF_HD void dft16(float2* v) {
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+    const float rr = 0.70710678118654752440f;
+    const float d = (float)DIR;
+    float2 a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        float2 q[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        dft4<DIR>(q);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = q[k1];
+    }
+    // twiddles W16^m, m = n2 k1 in {1,2,3,2,4,6,3,6,9}
+    const float2 w1 = c2(c1, d * s1), w2 = c2(rr, d * rr), w3 = c2(s1, d * c1);
+    const float2 w4 = c2(0.f, d), w6 = c2(-rr, d * rr), w9 = c2(-c1, -d * s1);
+    a[1][1] = cmul(a[1][1], w1);
+    a[1][2] = cmul(a[1][2], w2);
+    a[1][3] = cmul(a[1][3], w3);
+    a[2][1] = cmul(a[2][1], w2);
+    a[2][2] = cmul(a[2][2], w4);
+    a[2][3] = cmul(a[2][3], w6);
+    a[3][1] = cmul(a[3][1], w3);
+    a[3][2] = cmul(a[3][2], w6);
+    a[3][3] = cmul(a[3][3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 q[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+        dft4<DIR>(q);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = q[k2];
+    }
+}
+
+template <int R, int DIR>
+PF_HD void dft_any(float2* v) {
+    if constexpr (R == 16) dft16<DIR>(v);
+    else if constexpr (R == 7) {
+        // 7-point DFT directly (rare radix)
+        float2 x[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) x[i] = v[i];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            float2 acc = x[0];
+#pragma unroll
+            for (int n = 1; n < 7; ++n) {
+                const int m = (n * k) % 7;
+                const double a = (double)DIR * 2.0 * 3.14159265358979323846 * m / 7.0;
+                acc = cadd(acc, cmul(x[n], c2((float)__builtin_cos(a), (float)__builtin_sin(a))));
+            }
+            v[k] = acc;
+        }
+    } else dft_fixed<R, DIR>(v);
+}
+
+template <int N, int T, int DIR, int S>
+struct GroupStage {
+    static constexpr GPlan P = gplan_for(N, T);
+    static constexpr int E = N / T;
+    static constexpr bool kLast = S >= P.ns;
+
+    static constexpr int span() {
+        int ns = 1;
+        for (int i = 0; i < S; ++i) ns *= P.r[i];
+        return ns;
+    }
+
+    // Butterflies of this stage from the strided registers; outputs to ex at
+    // their natural positions.
+    static PF_HD void write_phase(const float2 (&v)[N / T], float2* ex, int es, int t,
+                                  const float2* __restrict__ tw) {
+        if constexpr (!kLast) {
+            constexpr int R = P.r[S];
+            constexpr int Ns = span();
+            constexpr int B = E / R;
+            static_assert(E % R == 0, "radix must divide elements per thread");
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = t + T * b;
+                const int k = j % Ns;
+                float2 u[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) u[r] = v[b + r * B];
+                if constexpr (Ns > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        u[r] = cmul(u[r], twid<DIR>(tw, r * k * (N / (Ns * R))));
+                }
+                dft_any<R, DIR>(u);
+                const int od = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) ex[(od + r * Ns) * es] = u[r];
+            }
+        }
+    }
+
+    // Next stage's inputs (strided layout) back from ex.
+    static PF_HD void read_phase(float2 (&v)[N / T], const float2* ex, int es, int t) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = ex[(t + T * e) * es];
+    }
+
+#if defined(__HIP__)
+    static __device__ __forceinline__ void run(float2 (&v)[N / T], float2* ex, int es, int t,
+                                               const float2* __restrict__ tw, bool active) {
+        if constexpr (!kLast) {
+            if (active) write_phase(v, ex, es, t, tw);
+            __syncthreads();
+            if (active) read_phase(v, ex, es, t);
+            __syncthreads();
+            GroupStage<N, T, DIR, S + 1>::run(v, ex, es, t, tw, active);
+        }
+    }
+#endif
+
+    // Sequential emulation of one group (all T threads) for host checks.
+    static void emulate(float2 (*v)[N / T], float2* ex, const float2* tw) {
+        if constexpr (!kLast) {
+            for (int t = 0; t < T; ++t) write_phase(v[t], ex, 1, t, tw);
+            for (int t = 0; t < T; ++t) read_phase(v[t], ex, 1, t);
+            GroupStage<N, T, DIR, S + 1>::emulate(v, ex, tw);
+        }
+    }
+};
+
+#if defined(__HIP__)
+// N-point FFT of the group's strided registers, exchanging through
+// ex[0 .. (N-1) * es] (N elements with element stride es).
+template <int N, int T, int DIR>
+__device__ __forceinline__ void group_fft(float2 (&v)[N / T], float2* ex, int es, int t,
+                                          const float2* __restrict__ tw, bool active) {
+    static_assert(gplan_for(N, T).ns >= 0, "no radix plan for this N and T");
+    GroupStage<N, T, DIR, 0>::run(v, ex, es, t, tw, active);
+}
+#endif
+
+}  // namespace pf

@@ -77,12 +77,38 @@ int launch_conj_multiply(const float2* a, const float2* b, float2* out, int64_t 
                          hipStream_t s);
 
 // Fused calc_lcc + take-best over `nrot` rotations whose unnormalised
-// correlations sit at real + (3 b + {0,1,2}) V.
-int launch_lcc_take_best(const float* real, const uint8_t* mask, float* best, int32_t* idx,
+// correlations sit at real + (3 b + {0,1,2}) V; merges into keys (lcc_key).
+int launch_lcc_take_best(const float* real, const uint8_t* mask, unsigned long long* keys,
                          int64_t V, int nrot, int64_t first, double norm, double invV,
                          hipStream_t s);
+int launch_fill_keys(unsigned long long* keys, int64_t n, unsigned long long value,
+                     hipStream_t s);
 int launch_calc_lcc(const float* gcc, const float* ave, const float* ave2,
                     const uint8_t* mask, float* lcc, int64_t n, hipStream_t s);
+
+// ---- fused search path (pf_fused.hip) ----
+// True when a compile-time fused pipeline exists for this grid.
+bool fused_supported(int nz, int ny, int nx);
+struct FusedArgs {
+    int n;                       // cubic grid edge
+    const float2* tm;            // packed (template, mask), V entries
+    const double* rots;          // rotation matrices of this batch (row-major 3x3)
+    int nrot;                    // rotations in this batch
+    int64_t first;               // global index of the first rotation
+    int rmax;
+    bool binary_mask;            // mask values all in {0, 1}: W^2 = W
+    float2* work;                // [nrot][3][H] spectra
+    const float2* F1;
+    const float2* F2;
+    const uint8_t* lcc_mask;
+    unsigned long long* keys;
+    double norm, invV;
+    int chunk_z, chunk_lcc;      // rotations per block in the z and LCC kernels
+};
+// Enqueue the three fused stages; `which` selects the stage (0: rotate +
+// forward xy, 1: z correlate, 2: inverse xy + LCC) or -1 for all three.
+int launch_fused(const FusedArgs& a, int which, hipStream_t s);
+int launch_pack_tm(const float* tmpl, const float* mask, float2* tm, int64_t n, hipStream_t s);
 
 // f64 -> f32 conversion, optionally also writing the square (computed in f64).
 int launch_convert(const double* in, float* out, float* out_sq, int64_t n, hipStream_t s);

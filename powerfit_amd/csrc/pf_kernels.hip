@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "pf_internal.h"
+#include "pf_rotate.h"
 
 namespace pf {
 
@@ -165,107 +166,6 @@ __global__ void k_c2c_axis(float2* __restrict__ data, int64_t stride, int64_t in
     }
 }
 
-// ---------------------------------------------------------------------------
-// Rotation (reference: _extensions.c:7-196).
-//
-// Gather form of the reference's scatter loop: output voxel (iz, iy, ix) takes
-// the value written last by the reference, i.e. from the lexicographically
-// largest signed triple (z, y, x), each congruent to its index and inside the
-// sphere |v|^2 <= r^2.  Source coordinates are evaluated in FP64 in the
-// reference's order without contraction; index arithmetic (floor/round, one
-// "+dim" wrap for negatives, the upper trilinear neighbour wrapping only at 0)
-// is the reference's.  Out-of-range source indices read 0.
-
-__device__ __forceinline__ int first_cand(int i, int n, int r) {
-    // largest s <= r with s == i (mod n)
-    int q = r - i;
-    int fl = q >= 0 ? q / n : -((-q + n - 1) / n);
-    return i + fl * n;
-}
-
-// Returns false when no in-sphere triple maps to this voxel.
-__device__ __forceinline__ bool signed_triple(int iz, int iy, int ix, int nz, int ny, int nx,
-                                              int r, int* z, int* y, int* x) {
-    const int r2 = r * r;
-    for (int sz = first_cand(iz, nz, r); sz >= -r; sz -= nz) {
-        const int d2z = sz * sz;
-        if (d2z > r2) continue;
-        for (int sy = first_cand(iy, ny, r); sy >= -r; sy -= ny) {
-            const int d2zy = d2z + sy * sy;
-            if (d2zy > r2) continue;
-            for (int sx = first_cand(ix, nx, r); sx >= -r; sx -= nx) {
-                if (d2zy + sx * sx <= r2) {
-                    *z = sz;
-                    *y = sy;
-                    *x = sx;
-                    return true;
-                }
-            }
-        }
-    }
-    return false;
-}
-
-template <typename T>
-__device__ __forceinline__ double rd(const T* g, int64_t i, int64_t size) {
-    return (i >= 0 && i < size) ? (double)g[i] : 0.0;
-}
-
-struct Coord {
-    double xs, ys, zs;
-};
-
-__device__ __forceinline__ Coord source_coord(const double* R, int z, int y, int x) {
-#pragma clang fp contract(off)
-    Coord c;
-    const double xz = R[6] * z, yz = R[7] * z, zz = R[8] * z;
-    const double xzy = xz + R[3] * y, yzy = yz + R[4] * y, zzy = zz + R[5] * y;
-    c.xs = xzy + R[0] * x;
-    c.ys = yzy + R[1] * x;
-    c.zs = zzy + R[2] * x;
-    return c;
-}
-
-template <typename T>
-__device__ __forceinline__ double sample_nearest(const T* g, int gz, int gy, int gx,
-                                                 const Coord& c) {
-    const int x0 = (int)round(c.xs), y0 = (int)round(c.ys), z0 = (int)round(c.zs);
-    const int64_t gslice = (int64_t)gy * gx, gsize = gslice * gz;
-    int64_t i = (int64_t)z0 * gslice + (int64_t)y0 * gx + x0;
-    if (x0 < 0) i += gx;
-    if (y0 < 0) i += gslice;
-    if (z0 < 0) i += gsize;
-    return rd(g, i, gsize);
-}
-
-template <typename T>
-__device__ __forceinline__ double sample_trilinear(const T* g, int gz, int gy, int gx,
-                                                   const Coord& c) {
-#pragma clang fp contract(off)
-    const int x0 = (int)floor(c.xs), y0 = (int)floor(c.ys), z0 = (int)floor(c.zs);
-    const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
-    const int64_t gslice = (int64_t)gy * gx, gsize = gslice * gz;
-    int64_t i = (int64_t)z0 * gslice + (int64_t)y0 * gx + x0;
-    if (x0 < 0) i += gx;
-    if (y0 < 0) i += gslice;
-    if (z0 < 0) i += gsize;
-    const double dx = c.xs - x0, dy = c.ys - y0, dz = c.zs - z0;
-    const double dx1 = 1 - dx, dy1 = 1 - dy, dz1 = 1 - dz;
-    const int64_t wx = (x1 == 0) ? -(int64_t)gx : 0;
-    const int64_t wy = (y1 == 0) ? -gslice : 0;
-    const int64_t wz = (z1 == 0) ? -gsize : 0;
-    const double c00 = rd(g, i, gsize) * dx1 + rd(g, i + 1 + wx, gsize) * dx;
-    const int64_t o10 = gx + wy;
-    const double c10 = rd(g, i + o10, gsize) * dx1 + rd(g, i + o10 + 1 + wx, gsize) * dx;
-    const int64_t o01 = gslice + wz;
-    const double c01 = rd(g, i + o01, gsize) * dx1 + rd(g, i + o01 + 1 + wx, gsize) * dx;
-    const int64_t o11 = gslice + gx + wz + wy;
-    const double c11 = rd(g, i + o11, gsize) * dx1 + rd(g, i + o11 + 1 + wx, gsize) * dx;
-    const double c0 = c00 * dy1 + c10 * dy;
-    const double c1 = c01 * dy1 + c11 * dy;
-    return c0 * dz1 + c1 * dz;
-}
-
 // Template (trilinear) and mask (nearest) of `nrot` rotations; blockIdx.y = rotation.
 __global__ void k_rotate_pw(const float* __restrict__ tmpl, const float* __restrict__ mask,
                             const double* __restrict__ rots, int nz, int ny, int nx, int r,
@@ -339,33 +239,35 @@ __global__ void k_conj_multiply(const float2* __restrict__ a, const float2* __re
 }
 
 // calc_lcc (reference: _powerfit.pyx:56-72, with ave2 *= N from
-// powerfitter.py:369) and take-best (powerfitter.py:327-330): strict '>',
-// ties keep the lower rotation index, NaN never wins.  The local variance
-// N<W^2 T^2> - <W T>^2 is formed in FP64.
+// powerfitter.py:369) and take-best (powerfitter.py:327-330): strict '>'
+// over the batch in rotation order, merged into the running result with
+// atomicMax on lcc_key (ties keep the lower rotation index, NaN never wins).
+// The local variance N<W^2 T^2> - <W T>^2 is formed in FP64.
 __global__ void k_lcc_take_best(const float* __restrict__ real, const uint8_t* __restrict__ mask,
-                                float* __restrict__ best, int32_t* __restrict__ idx, int64_t V,
-                                int nrot, int64_t first, double norm, double invV) {
+                                unsigned long long* __restrict__ keys, int64_t V, int nrot,
+                                int64_t first, double norm, double invV) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V || !mask[v]) return;
-    float bv = best[v];
-    int32_t bi = idx[v];
-    bool changed = false;
+    float bv = 0.f;
+    int32_t bi = -1;
     for (int b = 0; b < nrot; ++b) {
         const float* base = real + (int64_t)(3 * b) * V + v;
         const double g = (double)base[0] * invV;
         const double a = (double)base[V] * invV;
         const double a2 = (double)base[2 * V] * invV * norm;
         const float s = (float)(g / sqrt(a2 - a * a));
-        if (s > bv) {
+        if (bi < 0 ? (s == s) : (s > bv)) {
             bv = s;
             bi = (int32_t)(first + b);
-            changed = true;
         }
     }
-    if (changed) {
-        best[v] = bv;
-        idx[v] = bi;
-    }
+    if (bi >= 0) atomicMax(keys + v, lcc_key(bv, (uint32_t)bi));
+}
+
+__global__ void k_fill_keys(unsigned long long* __restrict__ keys, int64_t n,
+                            unsigned long long value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = value;
 }
 
 __global__ void k_calc_lcc(const float* __restrict__ gcc, const float* __restrict__ ave,
@@ -580,13 +482,20 @@ int launch_conj_multiply(const float2* a, const float2* b, float2* out, int64_t 
     return check_launch("k_conj_multiply");
 }
 
-int launch_lcc_take_best(const float* real, const uint8_t* mask, float* best, int32_t* idx,
+int launch_lcc_take_best(const float* real, const uint8_t* mask, unsigned long long* keys,
                          int64_t V, int nrot, int64_t first, double norm, double invV,
                          hipStream_t s) {
     if (V <= 0 || nrot <= 0) return kOk;
     hipLaunchKernelGGL(k_lcc_take_best, dim3(blocks_for(V)), dim3(kThreads), 0, s, real, mask,
-                       best, idx, V, nrot, first, norm, invV);
+                       keys, V, nrot, first, norm, invV);
     return check_launch("k_lcc_take_best");
+}
+
+int launch_fill_keys(unsigned long long* keys, int64_t n, unsigned long long value,
+                     hipStream_t s) {
+    if (n <= 0) return kOk;
+    hipLaunchKernelGGL(k_fill_keys, dim3(blocks_for(n)), dim3(kThreads), 0, s, keys, n, value);
+    return check_launch("k_fill_keys");
 }
 
 int launch_calc_lcc(const float* gcc, const float* ave, const float* ave2,

@@ -47,8 +47,8 @@ class PowerFitter:
         self._nproc = 1
         self._directory = abspath("./")
         self._laplace = laplace
-        self.mode = "batched"
-        self.batch = 16
+        self.mode = "auto"
+        self.batch = 64
 
     @property
     def directory(self):

@@ -150,3 +150,38 @@ def test_scan_before_setup_raises():
     corr = HipCorrelator(c.target.astype(np.float64))
     with pytest.raises(ValueError):
         corr.scan()
+
+
+FUSED_CASES = [("cpu64", 96), ("lc96", 24), ("search128", 12)]
+
+
+@pytest.mark.parametrize("mode", ["mix_fwd", "mix_z", "mix_inv", "fused"])
+@pytest.mark.parametrize("name,nrot", FUSED_CASES)
+def test_fused_stages_vs_oracle(name, nrot, mode):
+    """Each fused stage alone (the rest unfused), then all three fused."""
+    c, R, ref = case_and_oracle(name, nrot=nrot)
+    say(f"scan {name} mode={mode} first {nrot} rotations")
+    best, rot = run_hip(c, R, mode, 16)
+    err, dec = assert_scan_parity(best, rot, ref)
+    say(f"  max err {err:.2e}, decisive voxels {dec}")
+
+
+def test_fused_without_binary_shortcut():
+    c, R, ref = case_and_oracle("cpu64", nrot=96)
+    say("scan cpu64 fused, W^2 transformed separately")
+    corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused", batch=16,
+                         binary_shortcut=False)
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    corr.scan()
+    assert_scan_parity(corr.lcc, corr.rot, ref)
+
+
+def test_fused_batch_sizes_identical():
+    """Chunking and batch boundaries do not change any per-rotation value."""
+    c, R, ref = case_and_oracle("cpu64", nrot=96)
+    say("scan cpu64 fused batch 5 / 16 / 40")
+    outs = [run_hip(c, R, "fused", b) for b in (5, 16, 40)]
+    for other in outs[1:]:
+        assert np.array_equal(outs[0][0], other[0]) and np.array_equal(outs[0][1], other[1])
