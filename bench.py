@@ -89,6 +89,9 @@ def main(argv=None):
     ap.add_argument("--workload", default="search128")
     ap.add_argument("--plain-rotations", type=int, default=256,
                     help="rotations timed on the plain one-rotation-per-launch path")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 256 MiB device-to-device copy first (calibrates HBM byte "
+                         "counters under a profiler)")
     ap.add_argument("--cpu-per-worker", type=int, default=2,
                     help="rotations per CPU worker in the CPU baseline sample (0 = skip)")
     args = ap.parse_args(argv)
@@ -102,6 +105,14 @@ def main(argv=None):
     from powerfit_amd import HipCorrelator, _lib
     from powerfit_amd.rotations import rotations_for_angle
     from powerfit_amd.synth import WORKLOADS, make_case
+
+    if args.calib_copy:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     w = WORKLOADS[args.workload]
     case = make_case(w)

@@ -24,6 +24,7 @@ struct pf_ctx {
     float2* d_F2 = nullptr;       // F[Tf^2]
     uint8_t* d_lccmask = nullptr;
     float2* d_tm = nullptr;       // packed (template, mask) for the fused path
+    uint32_t* d_occ = nullptr;    // brick occupancy of d_tm (fused grids only)
     unsigned long long* d_keys = nullptr;  // running (LCC, index) keys
     uint32_t flags = 0;
     bool mask_binary = false;
@@ -265,6 +266,10 @@ int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** o
     if (!s) s = dev_alloc(&c->d_F2, c->H);
     if (!s) s = dev_alloc(&c->d_lccmask, c->V);
     if (!s) s = dev_alloc(&c->d_tm, c->V);
+    if (!s && fused_supported(nz, ny, nx)) {
+        const int64_t nb = nx / 8;
+        s = dev_alloc(&c->d_occ, (size_t)((nb * nb * nb + 31) / 32));
+    }
     if (!s) s = dev_alloc(&c->d_keys, c->V);
     if (s) {
         pf_ctx_destroy(c);
@@ -290,6 +295,7 @@ int pf_ctx_destroy(pf_ctx* c) {
         dev_free(c->d_F2);
         dev_free(c->d_lccmask);
         dev_free(c->d_tm);
+        dev_free(c->d_occ);
         dev_free(c->d_keys);
         dev_free(c->d_rot);
         dev_free(c->d_real);
@@ -363,6 +369,7 @@ int pf_set_template_mask(pf_ctx* c, const double* tmpl, const double* mask, doub
     }
     if (!s) s = launch_convert(dt, c->d_mask, nullptr, c->V, c->stream);
     if (!s) s = launch_pack_tm(c->d_template, c->d_mask, c->d_tm, c->V, c->stream);
+    if (!s && c->d_occ) s = launch_brick_occupancy(c->d_tm, c->nx, c->d_occ, c->stream);
     if (!s && hipStreamSynchronize(c->stream) != hipSuccess) {
         set_error("set_template_mask: stream sync failed");
         s = kErrHip;
@@ -427,6 +434,7 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     FusedArgs fa_args{};
     fa_args.n = c->nx;
     fa_args.tm = c->d_tm;
+    fa_args.occ = c->d_occ;
     fa_args.rmax = c->rmax;
     fa_args.work = c->d_cplx;
     fa_args.F1 = c->d_F1;

@@ -28,26 +28,44 @@
 
 namespace pf {
 
+// LDS row stride (complex elements) for M packed columns: >= M and = 8 mod 32,
+// so the rows one wave touches in the row FFTs fall on distinct bank groups.
+constexpr int padded_stride(int M) {
+    int s = M;
+    while (s % 32 != 8) ++s;
+    return s;
+}
+
 template <int N>
 struct Geo {
     static constexpr int M = N / 2;          // complex points of the half-length x FFT
     static constexpr int NXH = M + 1;        // kx extent in global memory
-    static constexpr int S = M + 1;          // LDS row stride (complex), padded
+    static constexpr int S = padded_stride(M);  // LDS row stride (complex)
+    static constexpr int NB = N / 8;         // 8^3 bricks per axis (occupancy map)
     static constexpr int PLANE = N * NXH;    // complex per z-plane in global memory
     static constexpr int64_t H = (int64_t)N * PLANE;
     static constexpr int64_t V = (int64_t)N * N * N;
 };
 
+// Copy a twiddle table into LDS (read there by the group FFTs: keeps the
+// tables out of registers).
+template <int L, int NTHR>
+__device__ __forceinline__ void stage_twiddles(float2* dst, const float2* __restrict__ src) {
+    for (int i = threadIdx.x; i < L; i += NTHR) dst[i] = src[i];
+}
+
 // Column (y) FFT of the M packed columns of an LDS plane, NTHR threads.
 template <int N, int NTHR, int DIR>
 __device__ __forceinline__ void plane_columns(float2* plane, const float2* __restrict__ twN) {
     using G = Geo<N>;
-    constexpr int TY = pick_T(N, 16, 32);
+    // enough threads per column that the M columns occupy the whole block
+    constexpr int TY = pick_T_min(N, 16, (NTHR + G::M - 1) / G::M, 32);
     constexpr int EY = N / TY;
     constexpr int CPI = NTHR / TY;
     constexpr int CITERS = (G::M + CPI - 1) / CPI;
-    static_assert(TY > 0, "no column plan");
-    const int gy = threadIdx.x / TY, ty = threadIdx.x % TY;
+    static_assert(TY > 0 && NTHR % TY == 0, "no column plan");
+    // column index fastest across lanes: a wave touches consecutive LDS words
+    const int gy = threadIdx.x % CPI, ty = threadIdx.x / CPI;
 #pragma unroll
     for (int ci = 0; ci < CITERS; ++ci) {
         const int c = ci * CPI + gy;
@@ -102,21 +120,77 @@ __device__ __forceinline__ void load_plane(float2* plane, const float2* __restri
     }
 }
 
+// Rotated template (trilinear) and mask (nearest) at signed voxel (z, y, x),
+// from the interleaved (template, mask) grid.  Same FP64 coordinates, floor,
+// round and corner indices as sample_trilinear / sample_nearest; the nearest
+// voxel is always one of the eight trilinear corners (round(s) is floor(s) or
+// floor(s) + 1, with the same wrap), so its mask value comes from the corner
+// loads.  Sources whose 2x2x2 corner cube lies in an all-zero region (occ
+// bit clear for the 8^3 brick of the lower corner, bricks dilated by one
+// voxel towards +x/+y/+z with wrap) are exactly zero and skip the loads.
+template <int N>
+__device__ __forceinline__ void gather_pw(const float2* __restrict__ tm, const uint32_t* occ,
+                                          const double* R, int z, int y, int x, float* p,
+                                          float* w) {
+#pragma clang fp contract(off)
+    using G = Geo<N>;
+    const Coord c = source_coord(R, z, y, x);
+    const int x0 = (int)floor(c.xs), y0 = (int)floor(c.ys), z0 = (int)floor(c.zs);
+    const int xi = x0 + (x0 < 0 ? N : 0), yi = y0 + (y0 < 0 ? N : 0), zi = z0 + (z0 < 0 ? N : 0);
+    *p = 0.f;
+    *w = 0.f;
+    if ((unsigned)xi >= (unsigned)N || (unsigned)yi >= (unsigned)N || (unsigned)zi >= (unsigned)N)
+        return;
+    const int brick = ((zi >> 3) * G::NB + (yi >> 3)) * G::NB + (xi >> 3);
+    if (!((occ[brick >> 5] >> (brick & 31)) & 1u)) return;
+    const int xj = (xi + 1 == N) ? 0 : xi + 1;
+    const int yj = (yi + 1 == N) ? 0 : yi + 1;
+    const int zj = (zi + 1 == N) ? 0 : zi + 1;
+    const float2* r00 = tm + (zi * N + yi) * N;
+    const float2* r10 = tm + (zi * N + yj) * N;
+    const float2* r01 = tm + (zj * N + yi) * N;
+    const float2* r11 = tm + (zj * N + yj) * N;
+    const float2 a000 = r00[xi], a001 = r00[xj], a010 = r10[xi], a011 = r10[xj];
+    const float2 a100 = r01[xi], a101 = r01[xj], a110 = r11[xi], a111 = r11[xj];
+    const double dx = c.xs - x0, dy = c.ys - y0, dz = c.zs - z0;
+    const double dx1 = 1 - dx, dy1 = 1 - dy, dz1 = 1 - dz;
+    const double c00 = (double)a000.x * dx1 + (double)a001.x * dx;
+    const double c10 = (double)a010.x * dx1 + (double)a011.x * dx;
+    const double c01 = (double)a100.x * dx1 + (double)a101.x * dx;
+    const double c11 = (double)a110.x * dx1 + (double)a111.x * dx;
+    const double c0 = c00 * dy1 + c10 * dy;
+    const double c1 = c01 * dy1 + c11 * dy;
+    *p = (float)(c0 * dz1 + c1 * dz);
+    const bool ux = (int)round(c.xs) != x0, uy = (int)round(c.ys) != y0,
+               uz = (int)round(c.zs) != z0;
+    const float2 n0 = uy ? (ux ? a011 : a010) : (ux ? a001 : a000);
+    const float2 n1 = uy ? (ux ? a111 : a110) : (ux ? a101 : a100);
+    *w = uz ? n1.y : n0.y;
+}
+
 // ---------------------------------------------------------------------------
 template <int N, int NTHR, bool BIN>
 __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ tm,
                                                     const double* __restrict__ rots, int r,
                                                     float2* __restrict__ out,
                                                     const float2* __restrict__ twM,
-                                                    const float2* __restrict__ twN) {
+                                                    const float2* __restrict__ twN,
+                                                    const uint32_t* __restrict__ occ_g) {
     using G = Geo<N>;
     constexpr int M = G::M;
     constexpr int TX = pick_T(M, 16, 16);
     constexpr int EX = M / TX;
     constexpr int RPI = NTHR / TX;
     constexpr int ITERS = (N + RPI - 1) / RPI;
+    constexpr int OCCW = (G::NB * G::NB * G::NB + 31) / 32;
     static_assert(TX > 0 && NTHR % TX == 0, "row plan");
     __shared__ float2 plane[N * G::S];
+    __shared__ uint32_t occ[OCCW];
+    __shared__ float2 stwM[M], stwN[N];
+    for (int i = threadIdx.x; i < OCCW; i += NTHR) occ[i] = occ_g[i];
+    stage_twiddles<M, NTHR>(stwM, twM);
+    stage_twiddles<N, NTHR>(stwN, twN);
+    __syncthreads();
 
     const int iz = blockIdx.x;
     const int b = blockIdx.y;
@@ -124,8 +198,6 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     double R[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
-    const float* tmpl = reinterpret_cast<const float*>(tm);
-
     float w[ITERS][EX][2];
     float2* dst = out + ((int64_t)b * 3) * G::H + (int64_t)iz * G::PLANE;
 
@@ -147,11 +219,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                         if (g == 0) {
                             int z, y, x;
                             float p = 0.f, wv = 0.f;
-                            if (signed_triple(iz, row, 2 * m + h, N, N, N, r, &z, &y, &x)) {
-                                const Coord c = source_coord(R, z, y, x);
-                                p = (float)sample_trilinear<2>(tmpl, N, N, N, c);
-                                wv = (float)sample_nearest<2>(tmpl + 1, N, N, N, c);
-                            }
+                            if (signed_triple(iz, row, 2 * m + h, N, N, N, r, &z, &y, &x))
+                                gather_pw<N>(tm, occ, R, z, y, x, &p, &wv);
                             pv[h] = p;
                             w[it][e][h] = wv;
                         } else if (g == 1) {
@@ -164,7 +233,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                 }
             }
             float2* rowp = plane + row * G::S;
-            group_fft<M, TX, -1>(v, active ? rowp : plane, 1, tx, twM, active);
+            group_fft<M, TX, -1>(v, active ? rowp : plane, 1, tx, stwM, active);
             // R2C post-processing: X[k] = E[k] + w^k O[k]; X0, XM packed in column 0
             if (active) {
 #pragma unroll
@@ -184,7 +253,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                         const float2 Ev = cscale(cadd(zk, zm), 0.5f);
                         const float2 Od = mul_si<-1>(cscale(csub(zk, zm), 0.5f));
                         // exp(-2 pi i k / N) = twN[k]
-                        X[e] = cadd(Ev, cmul(twN[k], Od));
+                        X[e] = cadd(Ev, cmul(stwN[k], Od));
                     }
                 }
             }
@@ -195,7 +264,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
             }
         }
         __syncthreads();
-        plane_columns<N, NTHR, -1>(plane, twN);
+        plane_columns<N, NTHR, -1>(plane, stwN);
         store_plane<N, NTHR>(plane, dst + (int64_t)g * G::H);
         __syncthreads();
     }
@@ -210,6 +279,9 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
     constexpr int TZ = pick_T(N, EMAX, 32);
     constexpr int EZ = N / TZ;
     __shared__ float2 ex[N * TQ];
+    __shared__ float2 stw[N];
+    stage_twiddles<N, TQ * TZ>(stw, twN);
+    __syncthreads();
     const int c = threadIdx.x % TQ, t = threadIdx.x / TQ;
     const int q = blockIdx.x * TQ + c;
     const bool active = q < G::PLANE;
@@ -234,7 +306,7 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
 #pragma unroll
                 for (int e = 0; e < EZ; ++e) v[e] = col[(int64_t)(t + TZ * e) * G::PLANE];
             }
-            group_fft<N, TZ, -1>(v, exc, TQ, t, twN, active);
+            group_fft<N, TZ, -1>(v, exc, TQ, t, stw, active);
             if (BIN && g == 1) {
                 // W spectrum feeds both A (with F1) and B (with F2)
                 float2 u[EZ];
@@ -242,8 +314,8 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
                 for (int e = 0; e < EZ; ++e) u[e] = cmul(cconj(v[e]), f2[e]);
 #pragma unroll
                 for (int e = 0; e < EZ; ++e) v[e] = cmul(cconj(v[e]), f1[e]);
-                group_fft<N, TZ, 1>(v, exc, TQ, t, twN, active);
-                group_fft<N, TZ, 1>(u, exc, TQ, t, twN, active);
+                group_fft<N, TZ, 1>(v, exc, TQ, t, stw, active);
+                group_fft<N, TZ, 1>(u, exc, TQ, t, stw, active);
                 if (active) {
 #pragma unroll
                     for (int e = 0; e < EZ; ++e) {
@@ -254,7 +326,7 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
             } else {
 #pragma unroll
                 for (int e = 0; e < EZ; ++e) v[e] = cmul(cconj(v[e]), g == 2 ? f2[e] : f1[e]);
-                group_fft<N, TZ, 1>(v, exc, TQ, t, twN, active);
+                group_fft<N, TZ, 1>(v, exc, TQ, t, stw, active);
                 if (active) {
 #pragma unroll
                     for (int e = 0; e < EZ; ++e) col[(int64_t)(t + TZ * e) * G::PLANE] = v[e];
@@ -281,11 +353,9 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
     constexpr int ITERS = (N + RPI - 1) / RPI;
     static_assert(ITERS * EX * 2 <= 32, "owned voxels per thread must fit a 32-bit mask");
     __shared__ float2 plane[N * G::S];
-    // chunk-local running max (strict '>') and its rotation offset, per voxel
-    // of the plane; 0xff = no finite value seen yet.  Each entry is touched
-    // only by the thread owning that voxel.
-    __shared__ float sbest[N * N];
-    __shared__ uint8_t sidx[N * N];
+    __shared__ float2 stwM[M], stwN[N];
+    stage_twiddles<M, NTHR>(stwM, twM);
+    stage_twiddles<N, NTHR>(stwN, twN);
 
     const int iz = blockIdx.x;
     const int b0 = blockIdx.y * chunk;
@@ -300,17 +370,13 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
     for (int it = 0; it < ITERS; ++it) {
         const int row = it * RPI + gx;
 #pragma unroll
-        for (int e = 0; e < EX; ++e) {
+        for (int e = 0; e < EX; ++e)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < 2; ++h)
                 if (row < N) {
-                    const int x = 2 * (tx + TX * e) + h;
-                    const int64_t vox = ((int64_t)iz * N + row) * N + x;
+                    const int64_t vox = ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
                     if (lmask[vox]) mbits |= 1u << (2 * (it * EX + e) + h);
-                    sidx[row * N + x] = 0xff;
                 }
-            }
-        }
     }
 
     for (int b = b0; b < b1; ++b) {
@@ -319,7 +385,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
             const int g = 2 - gi;   // B (ave2), then A (ave), then G (gcc)
             load_plane<N, NTHR>(plane, X + ((int64_t)b * 3 + g) * G::H + (int64_t)iz * G::PLANE);
             __syncthreads();
-            plane_columns<N, NTHR, 1>(plane, twN);
+            plane_columns<N, NTHR, 1>(plane, stwN);
 #pragma unroll
             for (int it = 0; it < ITERS; ++it) {
                 const int row = it * RPI + gx;
@@ -334,12 +400,12 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
                         const float2 xk = (k == 0) ? c2(c0.x, 0.f) : rowp[k];
                         const float2 xm = (k == 0) ? c2(c0.y, 0.f) : rowp[M - k];
                         const float2 xmc = cconj(xm);
-                        const float2 d = cmul(csub(xk, xmc), cconj(twN[k]));
+                        const float2 d = cmul(csub(xk, xmc), cconj(stwN[k]));
                         v[e] = cadd(cadd(xk, xmc), mul_si<1>(d));
                     }
                 }
                 __syncthreads();
-                group_fft<M, TX, 1>(v, rowp, 1, tx, twM, active);
+                group_fft<M, TX, 1>(v, rowp, 1, tx, stwM, active);
                 if (active) {
 #pragma unroll
                     for (int e = 0; e < EX; ++e) {
@@ -354,12 +420,12 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
                                 const double a2 = (double)isd[it][e][h] * invV * norm;
                                 isd[it][e][h] = rsqrtf((float)(a2 - a * a));
                             } else if ((mbits >> (2 * (it * EX + e) + h)) & 1u) {
-                                const float s = val * invVf * isd[it][e][h];
-                                const int li = row * N + 2 * (tx + TX * e) + h;
-                                const uint8_t bi = sidx[li];
-                                if (bi == 0xff ? (s == s) : (s > sbest[li])) {
-                                    sbest[li] = s;
-                                    sidx[li] = (uint8_t)(b - b0);
+                                // only S > 0 can beat the initial (0, 0); NaN never wins
+                                const float sv = val * invVf * isd[it][e][h];
+                                if (sv > 0.f) {
+                                    const int64_t vox =
+                                        ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
+                                    atomicMax(keys + vox, lcc_key(sv, (uint32_t)(first + b)));
                                 }
                             }
                         }
@@ -369,23 +435,28 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
             __syncthreads();
         }
     }
-#pragma unroll
-    for (int it = 0; it < ITERS; ++it) {
-        const int row = it * RPI + gx;
-#pragma unroll
-        for (int e = 0; e < EX; ++e)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                if (row >= N) continue;
-                const int x = 2 * (tx + TX * e) + h;
-                const int li = row * N + x;
-                if (sidx[li] != 0xff) {
-                    const int64_t vox = ((int64_t)iz * N + row) * N + x;
-                    atomicMax(keys + vox,
-                              lcc_key(sbest[li], (uint32_t)(first + b0 + sidx[li])));
+}
+
+// Brick occupancy for gather_pw: bit b set when any voxel of 8^3 brick b, or
+// of the one-voxel layer after it in +x/+y/+z (periodic), has a nonzero
+// template or mask value.
+__global__ void k_brick_occupancy(const float2* __restrict__ tm, int n, uint32_t* __restrict__ occ) {
+    const int nb = n / 8;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb * nb * nb) return;
+    const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+    bool any = false;
+    for (int dz = 0; dz <= 8 && !any; ++dz)
+        for (int dy = 0; dy <= 8 && !any; ++dy)
+            for (int dx = 0; dx <= 8; ++dx) {
+                const int z = (bz * 8 + dz) % n, y = (by * 8 + dy) % n, x = (bx * 8 + dx) % n;
+                const float2 v = tm[((int64_t)z * n + y) * n + x];
+                if (v.x != 0.f || v.y != 0.f) {
+                    any = true;
+                    break;
                 }
             }
-    }
+    if (any) atomicOr(occ + (b >> 5), 1u << (b & 31));
 }
 
 __global__ void k_pack_tm(const float* __restrict__ t, const float* __restrict__ m,
@@ -426,10 +497,10 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     if (which < 0 || which == 0) {
         if (a.binary_mask)
             hipLaunchKernelGGL((k_fwd_plane<N, NTHR1, true>), dim3(N, a.nrot), dim3(NTHR1), 0, s,
-                               a.tm, a.rots, a.rmax, a.work, tw.M, tw.N);
+                               a.tm, a.rots, a.rmax, a.work, tw.M, tw.N, a.occ);
         else
             hipLaunchKernelGGL((k_fwd_plane<N, NTHR1, false>), dim3(N, a.nrot), dim3(NTHR1), 0, s,
-                               a.tm, a.rots, a.rmax, a.work, tw.M, tw.N);
+                               a.tm, a.rots, a.rmax, a.work, tw.M, tw.N, a.occ);
         PF_CHECK_STATUS(last("k_fwd_plane"));
     }
     if (which < 0 || which == 1) {
@@ -477,11 +548,22 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
     switch (a.n) {
         case 64: return run_fused<64, 256, 32, 8, 256>(a, which, s);
         case 96: return run_fused<96, 384, 32, 12, 384>(a, which, s);
-        case 128: return run_fused<128, 512, 32, 8, 512>(a, which, s);
+        case 128: return run_fused<128, 1024, 32, 8, 1024>(a, which, s);
         default:
             set_error("fused pipeline not built for n = %d", a.n);
             return kErrUnsupported;
     }
+}
+
+int launch_brick_occupancy(const float2* tm, int n, uint32_t* occ, hipStream_t s) {
+    if (n % 8 != 0) {
+        set_error("brick occupancy needs n divisible by 8");
+        return kErrArg;
+    }
+    const int nb = n / 8, nbricks = nb * nb * nb;
+    PF_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(uint32_t) * ((nbricks + 31) / 32), s));
+    hipLaunchKernelGGL(k_brick_occupancy, dim3((nbricks + 255) / 256), dim3(256), 0, s, tm, n, occ);
+    return last("k_brick_occupancy");
 }
 
 int launch_pack_tm(const float* tmpl, const float* mask, float2* tm, int64_t n, hipStream_t s) {

@@ -68,6 +68,22 @@ constexpr int pick_T(int N, int emax, int tmax) {
     return best_t;
 }
 
+// As pick_T but with at least tmin threads per transform (falls back to
+// pick_T(N, emax, tmax) when no such plan exists).
+constexpr int pick_T_min(int N, int emax, int tmin, int tmax) {
+    int best_t = 0, best_ns = 100;
+    for (int T = tmax; T >= tmin && T >= 1; --T) {
+        if (N % T != 0 || N / T > emax) continue;
+        GPlan p = gplan_for(N, T);
+        if (p.ns < 0) continue;
+        if (p.ns < best_ns) {
+            best_ns = p.ns;
+            best_t = T;
+        }
+    }
+    return best_t ? best_t : pick_T(N, emax, tmax);
+}
+
 // 16-point DFT as 4 x 4 with inter-stage twiddles W16^(n2 k1).
 template <int DIR>
 PF_HD void dft16(float2* v) {

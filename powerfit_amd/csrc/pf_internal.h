@@ -92,6 +92,7 @@ bool fused_supported(int nz, int ny, int nx);
 struct FusedArgs {
     int n;                       // cubic grid edge
     const float2* tm;            // packed (template, mask), V entries
+    const uint32_t* occ;         // brick occupancy of tm (launch_brick_occupancy)
     const double* rots;          // rotation matrices of this batch (row-major 3x3)
     int nrot;                    // rotations in this batch
     int64_t first;               // global index of the first rotation
@@ -109,6 +110,8 @@ struct FusedArgs {
 // forward xy, 1: z correlate, 2: inverse xy + LCC) or -1 for all three.
 int launch_fused(const FusedArgs& a, int which, hipStream_t s);
 int launch_pack_tm(const float* tmpl, const float* mask, float2* tm, int64_t n, hipStream_t s);
+// 1 bit per 8^3 brick of an n^3 interleaved grid (n % 8 == 0): ((n/8)^3 + 31) / 32 words.
+int launch_brick_occupancy(const float2* tm, int n, uint32_t* occ, hipStream_t s);
 
 // f64 -> f32 conversion, optionally also writing the square (computed in f64).
 int launch_convert(const double* in, float* out, float* out_sq, int64_t n, hipStream_t s);
