@@ -36,11 +36,17 @@ constexpr int padded_stride(int M) {
     return s;
 }
 
+// Row FFTs (groups of consecutive lanes, one wavefront) synchronise their
+// stages per wavefront; column FFTs run lanes across columns (conflict-free
+// LDS) with workgroup barriers.
+constexpr bool kWaveRows = true;
+constexpr bool kWaveCols = false;
+
 template <int N>
 struct Geo {
     static constexpr int M = N / 2;          // complex points of the half-length x FFT
     static constexpr int NXH = M + 1;        // kx extent in global memory
-    static constexpr int S = padded_stride(M);  // LDS row stride (complex)
+    static constexpr int S = padded_stride(M + 1);  // LDS row stride (complex), >= M + 1
     static constexpr int NB = N / 8;         // 8^3 bricks per axis (occupancy map)
     static constexpr int PLANE = N * NXH;    // complex per z-plane in global memory
     static constexpr int64_t H = (int64_t)N * PLANE;
@@ -55,7 +61,12 @@ __device__ __forceinline__ void stage_twiddles(float2* dst, const float2* __rest
 }
 
 // Column (y) FFT of the M packed columns of an LDS plane, NTHR threads.
-template <int N, int NTHR, int DIR>
+// WAVE: each column's threads are consecutive lanes of one wavefront and the
+// stages synchronise per wave; otherwise lanes run across columns (a wave
+// touches consecutive words) and stages use workgroup barriers.
+// PACK_IN: the plane holds the M+1 unpacked columns (DC at 0, Nyquist at M);
+// column 0 is transformed as DC + i Nyquist.
+template <int N, int NTHR, int DIR, bool WAVE, bool PACK_IN = false>
 __device__ __forceinline__ void plane_columns(float2* plane, const float2* __restrict__ twN) {
     using G = Geo<N>;
     // enough threads per column that the M columns occupy the whole block
@@ -64,8 +75,8 @@ __device__ __forceinline__ void plane_columns(float2* plane, const float2* __res
     constexpr int CPI = NTHR / TY;
     constexpr int CITERS = (G::M + CPI - 1) / CPI;
     static_assert(TY > 0 && NTHR % TY == 0, "no column plan");
-    // column index fastest across lanes: a wave touches consecutive LDS words
-    const int gy = threadIdx.x % CPI, ty = threadIdx.x / CPI;
+    const int gy = WAVE ? threadIdx.x / TY : threadIdx.x % CPI;
+    const int ty = WAVE ? threadIdx.x % TY : threadIdx.x / CPI;
 #pragma unroll
     for (int ci = 0; ci < CITERS; ++ci) {
         const int c = ci * CPI + gy;
@@ -73,10 +84,14 @@ __device__ __forceinline__ void plane_columns(float2* plane, const float2* __res
         float2 v[EY];
         if (active) {
 #pragma unroll
-            for (int e = 0; e < EY; ++e) v[e] = plane[(ty + TY * e) * G::S + c];
+            for (int e = 0; e < EY; ++e) {
+                v[e] = plane[(ty + TY * e) * G::S + c];
+                if (PACK_IN && c == 0)
+                    v[e] = cadd(v[e], mul_si<1>(plane[(ty + TY * e) * G::S + G::M]));
+            }
         }
         __syncthreads();
-        group_fft<N, TY, DIR>(v, plane + (active ? c : 0), G::S, ty, twN, active);
+        group_fft<N, TY, DIR, WAVE>(v, plane + (active ? c : 0), G::S, ty, twN, active);
         if (active) {
 #pragma unroll
             for (int e = 0; e < EY; ++e) plane[(ty + TY * e) * G::S + c] = v[e];
@@ -233,7 +248,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                 }
             }
             float2* rowp = plane + row * G::S;
-            group_fft<M, TX, -1>(v, active ? rowp : plane, 1, tx, stwM, active);
+            group_fft<M, TX, -1, kWaveRows>(v, active ? rowp : plane, 1, tx, stwM, active);
             // R2C post-processing: X[k] = E[k] + w^k O[k]; X0, XM packed in column 0
             if (active) {
 #pragma unroll
@@ -264,7 +279,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
             }
         }
         __syncthreads();
-        plane_columns<N, NTHR, -1>(plane, stwN);
+        plane_columns<N, NTHR, -1, kWaveCols>(plane, stwN);
         store_plane<N, NTHR>(plane, dst + (int64_t)g * G::H);
         __syncthreads();
     }
@@ -351,6 +366,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
     constexpr int EX = M / TX;
     constexpr int RPI = NTHR / TX;
     constexpr int ITERS = (N + RPI - 1) / RPI;
+    constexpr int PER = (G::PLANE + NTHR - 1) / NTHR;   // plane elements per thread
     static_assert(ITERS * EX * 2 <= 32, "owned voxels per thread must fit a 32-bit mask");
     __shared__ float2 plane[N * G::S];
     __shared__ float2 stwM[M], stwN[N];
@@ -379,60 +395,89 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
                 }
     }
 
-    for (int b = b0; b < b1; ++b) {
+    // Planes are visited in the order (b, B), (b, A), (b, G), (b + 1, B), ...;
+    // the next plane's global loads are issued into registers before the
+    // current one is transformed, so their latency overlaps the FFTs.
+    const int nplanes = 3 * (b1 - b0);
+    auto src_of = [&](int p) {
+        const int b = b0 + p / 3, g = 2 - p % 3;
+        return X + ((int64_t)b * 3 + g) * G::H + (int64_t)iz * G::PLANE;
+    };
+    float2 pf[PER];
+    if (nplanes > 0) {
+        const float2* src = src_of(0);
 #pragma unroll
-        for (int gi = 0; gi < 3; ++gi) {
-            const int g = 2 - gi;   // B (ave2), then A (ave), then G (gcc)
-            load_plane<N, NTHR>(plane, X + ((int64_t)b * 3 + g) * G::H + (int64_t)iz * G::PLANE);
-            __syncthreads();
-            plane_columns<N, NTHR, 1>(plane, stwN);
+        for (int i = 0; i < PER; ++i) {
+            const int f = threadIdx.x + NTHR * i;
+            if (f < G::PLANE) pf[i] = src[f];
+        }
+    }
+    for (int p = 0; p < nplanes; ++p) {
+        const int b = b0 + p / 3, g = 2 - p % 3;
+        __syncthreads();   // previous plane's row phase is done with the LDS
 #pragma unroll
-            for (int it = 0; it < ITERS; ++it) {
-                const int row = it * RPI + gx;
-                const bool active = row < N;
-                float2* rowp = plane + (active ? row : 0) * G::S;
-                float2 v[EX];
-                if (active) {
-                    const float2 c0 = rowp[0];   // (X0, XM), both real
+        for (int i = 0; i < PER; ++i) {
+            const int f = threadIdx.x + NTHR * i;
+            if (f < G::PLANE) {
+                const int ky = f / G::NXH, kx = f - ky * G::NXH;
+                plane[ky * G::S + kx] = pf[i];
+            }
+        }
+        if (p + 1 < nplanes) {
+            const float2* src = src_of(p + 1);
 #pragma unroll
-                    for (int e = 0; e < EX; ++e) {
-                        const int k = tx + TX * e;
-                        const float2 xk = (k == 0) ? c2(c0.x, 0.f) : rowp[k];
-                        const float2 xm = (k == 0) ? c2(c0.y, 0.f) : rowp[M - k];
-                        const float2 xmc = cconj(xm);
-                        const float2 d = cmul(csub(xk, xmc), cconj(stwN[k]));
-                        v[e] = cadd(cadd(xk, xmc), mul_si<1>(d));
-                    }
+            for (int i = 0; i < PER; ++i) {
+                const int f = threadIdx.x + NTHR * i;
+                if (f < G::PLANE) pf[i] = src[f];
+            }
+        }
+        __syncthreads();
+        plane_columns<N, NTHR, 1, kWaveCols, true>(plane, stwN);
+#pragma unroll
+        for (int it = 0; it < ITERS; ++it) {
+            const int row = it * RPI + gx;
+            const bool active = row < N;
+            float2* rowp = plane + (active ? row : 0) * G::S;
+            float2 v[EX];
+            if (active) {
+                const float2 c0 = rowp[0];   // (X0, XM), both real
+#pragma unroll
+                for (int e = 0; e < EX; ++e) {
+                    const int k = tx + TX * e;
+                    const float2 xk = (k == 0) ? c2(c0.x, 0.f) : rowp[k];
+                    const float2 xm = (k == 0) ? c2(c0.y, 0.f) : rowp[M - k];
+                    const float2 xmc = cconj(xm);
+                    const float2 d = cmul(csub(xk, xmc), cconj(stwN[k]));
+                    v[e] = cadd(cadd(xk, xmc), mul_si<1>(d));
                 }
-                __syncthreads();
-                group_fft<M, TX, 1>(v, rowp, 1, tx, stwM, active);
-                if (active) {
+            }
+            __syncthreads();
+            group_fft<M, TX, 1, kWaveRows>(v, rowp, 1, tx, stwM, active);
+            if (active) {
 #pragma unroll
-                    for (int e = 0; e < EX; ++e) {
+                for (int e = 0; e < EX; ++e) {
 #pragma unroll
-                        for (int h = 0; h < 2; ++h) {
-                            const float val = h ? v[e].y : v[e].x;
-                            if (g == 2) {
-                                isd[it][e][h] = val;
-                            } else if (g == 1) {
-                                // N<W^2 T^2> - <W T>^2 in FP64, then 1/sqrt in FP32
-                                const double a = (double)val * invV;
-                                const double a2 = (double)isd[it][e][h] * invV * norm;
-                                isd[it][e][h] = rsqrtf((float)(a2 - a * a));
-                            } else if ((mbits >> (2 * (it * EX + e) + h)) & 1u) {
-                                // only S > 0 can beat the initial (0, 0); NaN never wins
-                                const float sv = val * invVf * isd[it][e][h];
-                                if (sv > 0.f) {
-                                    const int64_t vox =
-                                        ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
-                                    atomicMax(keys + vox, lcc_key(sv, (uint32_t)(first + b)));
-                                }
+                    for (int h = 0; h < 2; ++h) {
+                        const float val = h ? v[e].y : v[e].x;
+                        if (g == 2) {
+                            isd[it][e][h] = val;
+                        } else if (g == 1) {
+                            // N<W^2 T^2> - <W T>^2 in FP64, then 1/sqrt in FP32
+                            const double a = (double)val * invV;
+                            const double a2 = (double)isd[it][e][h] * invV * norm;
+                            isd[it][e][h] = rsqrtf((float)(a2 - a * a));
+                        } else if ((mbits >> (2 * (it * EX + e) + h)) & 1u) {
+                            // only S > 0 can beat the initial (0, 0); NaN never wins
+                            const float sv = val * invVf * isd[it][e][h];
+                            if (sv > 0.f) {
+                                const int64_t vox =
+                                    ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
+                                atomicMax(keys + vox, lcc_key(sv, (uint32_t)(first + b)));
                             }
                         }
                     }
                 }
             }
-            __syncthreads();
         }
     }
 }

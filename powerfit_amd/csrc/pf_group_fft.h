@@ -141,7 +141,24 @@ PF_HD void dft_any(float2* v) {
     } else dft_fixed<R, DIR>(v);
 }
 
-template <int N, int T, int DIR, int S>
+#if defined(__HIP__)
+// Synchronisation between the write and read phases of a stage.  WAVE: all
+// threads of every group live in one wavefront, whose LDS operations execute
+// in issue order, so a wavefront-scope fence (stops compiler reordering)
+// suffices; otherwise a workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void group_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+#endif
+
+template <int N, int T, int DIR, int S, bool WAVE = false>
 struct GroupStage {
     static constexpr GPlan P = gplan_for(N, T);
     static constexpr int E = N / T;
@@ -193,10 +210,10 @@ struct GroupStage {
                                                const float2* __restrict__ tw, bool active) {
         if constexpr (!kLast) {
             if (active) write_phase(v, ex, es, t, tw);
-            __syncthreads();
+            group_sync<WAVE>();
             if (active) read_phase(v, ex, es, t);
-            __syncthreads();
-            GroupStage<N, T, DIR, S + 1>::run(v, ex, es, t, tw, active);
+            group_sync<WAVE>();
+            GroupStage<N, T, DIR, S + 1, WAVE>::run(v, ex, es, t, tw, active);
         }
     }
 #endif
@@ -206,19 +223,22 @@ struct GroupStage {
         if constexpr (!kLast) {
             for (int t = 0; t < T; ++t) write_phase(v[t], ex, 1, t, tw);
             for (int t = 0; t < T; ++t) read_phase(v[t], ex, 1, t);
-            GroupStage<N, T, DIR, S + 1>::emulate(v, ex, tw);
+            GroupStage<N, T, DIR, S + 1, WAVE>::emulate(v, ex, tw);
         }
     }
 };
 
 #if defined(__HIP__)
 // N-point FFT of the group's strided registers, exchanging through
-// ex[0 .. (N-1) * es] (N elements with element stride es).
-template <int N, int T, int DIR>
+// ex[0 .. (N-1) * es] (N elements with element stride es).  WAVE = true
+// requires every group's T threads to lie in one wavefront (T divides 64 and
+// groups are formed from consecutive lanes).
+template <int N, int T, int DIR, bool WAVE = false>
 __device__ __forceinline__ void group_fft(float2 (&v)[N / T], float2* ex, int es, int t,
                                           const float2* __restrict__ tw, bool active) {
     static_assert(gplan_for(N, T).ns >= 0, "no radix plan for this N and T");
-    GroupStage<N, T, DIR, 0>::run(v, ex, es, t, tw, active);
+    static_assert(!WAVE || (64 % T == 0), "wave-local groups need T | 64");
+    GroupStage<N, T, DIR, 0, WAVE>::run(v, ex, es, t, tw, active);
 }
 #endif
 
