@@ -415,7 +415,7 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     if (first < 0 || count < 0 || first + count > c->nrot)
         return arg_error("rotation range outside the rotation set");
     if (mode < PF_MODE_PLAIN || mode > PF_MODE_MIX_INV) return arg_error("unknown mode");
-    const bool fused_ok = fused_supported(c->nz, c->ny, c->nx);
+    const bool fused_ok = fused_supported(c->nz, c->ny, c->nx) && c->rmax <= c->nx / 2;
     if (mode == PF_MODE_AUTO) mode = fused_ok ? PF_MODE_FUSED : PF_MODE_BATCHED;
     if (mode >= PF_MODE_FUSED && !fused_ok) {
         set_error("fused pipeline not available for grid (%d, %d, %d)", c->nz, c->ny, c->nx);
@@ -506,7 +506,7 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
 
 int pf_ctx_fused_available(pf_ctx* c) {
     if (!c) return arg_error("ctx is null");
-    return fused_supported(c->nz, c->ny, c->nx) ? 1 : 0;
+    return (fused_supported(c->nz, c->ny, c->nx) && (c->rmax < 0 || c->rmax <= c->nx / 2)) ? 1 : 0;
 }
 
 int pf_get_result(pf_ctx* c, float* best, int32_t* idx) {

@@ -135,22 +135,32 @@ __device__ __forceinline__ void load_plane(float2* plane, const float2* __restri
     }
 }
 
+// Signed coordinate of index i on an axis of length N for rotation radius
+// r <= N / 2: the value the reference's scatter loop writes last, i.e. i for
+// i <= r (for i == r == N / 2 both +r and -r map here and +r comes later),
+// otherwise i - N.
+template <int N>
+__device__ __forceinline__ int signed_coord(int i, int r) {
+    return i <= r ? i : i - N;
+}
+
 // Rotated template (trilinear) and mask (nearest) at signed voxel (z, y, x),
-// from the interleaved (template, mask) grid.  Same FP64 coordinates, floor,
-// round and corner indices as sample_trilinear / sample_nearest; the nearest
-// voxel is always one of the eight trilinear corners (round(s) is floor(s) or
-// floor(s) + 1, with the same wrap), so its mask value comes from the corner
-// loads.  Sources whose 2x2x2 corner cube lies in an all-zero region (occ
-// bit clear for the 8^3 brick of the lower corner, bricks dilated by one
-// voxel towards +x/+y/+z with wrap) are exactly zero and skip the loads.
+// from the interleaved (template, mask) grid.  Coordinates, floor and round
+// are FP64 in the reference's evaluation order (so corner and nearest indices
+// are the reference's; the nearest voxel is always one of the eight trilinear
+// corners, since round(s) is floor(s) or floor(s) + 1 with the same wrap); the
+// interpolation itself runs in FP32 on the FP32 template.  Sources whose
+// 2x2x2 corner cube lies in an all-zero region (occ bit clear for the 8^3
+// brick of the lower corner, bricks dilated by one voxel towards +x/+y/+z
+// with wrap) are exactly zero and skip the loads.
 template <int N>
 __device__ __forceinline__ void gather_pw(const float2* __restrict__ tm, const uint32_t* occ,
                                           const double* R, int z, int y, int x, float* p,
                                           float* w) {
-#pragma clang fp contract(off)
     using G = Geo<N>;
-    const Coord c = source_coord(R, z, y, x);
-    const int x0 = (int)floor(c.xs), y0 = (int)floor(c.ys), z0 = (int)floor(c.zs);
+    const Coord c = source_coord(R, z, y, x);   // FP64, contraction off inside
+    const double fx = floor(c.xs), fy = floor(c.ys), fz = floor(c.zs);
+    const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
     const int xi = x0 + (x0 < 0 ? N : 0), yi = y0 + (y0 < 0 ? N : 0), zi = z0 + (z0 < 0 ? N : 0);
     *p = 0.f;
     *w = 0.f;
@@ -167,17 +177,20 @@ __device__ __forceinline__ void gather_pw(const float2* __restrict__ tm, const u
     const float2* r11 = tm + (zj * N + yj) * N;
     const float2 a000 = r00[xi], a001 = r00[xj], a010 = r10[xi], a011 = r10[xj];
     const float2 a100 = r01[xi], a101 = r01[xj], a110 = r11[xi], a111 = r11[xj];
-    const double dx = c.xs - x0, dy = c.ys - y0, dz = c.zs - z0;
-    const double dx1 = 1 - dx, dy1 = 1 - dy, dz1 = 1 - dz;
-    const double c00 = (double)a000.x * dx1 + (double)a001.x * dx;
-    const double c10 = (double)a010.x * dx1 + (double)a011.x * dx;
-    const double c01 = (double)a100.x * dx1 + (double)a101.x * dx;
-    const double c11 = (double)a110.x * dx1 + (double)a111.x * dx;
-    const double c0 = c00 * dy1 + c10 * dy;
-    const double c1 = c01 * dy1 + c11 * dy;
-    *p = (float)(c0 * dz1 + c1 * dz);
-    const bool ux = (int)round(c.xs) != x0, uy = (int)round(c.ys) != y0,
-               uz = (int)round(c.zs) != z0;
+    const float dx = (float)(c.xs - fx), dy = (float)(c.ys - fy), dz = (float)(c.zs - fz);
+    const float c00 = a000.x + (a001.x - a000.x) * dx;
+    const float c10 = a010.x + (a011.x - a010.x) * dx;
+    const float c01 = a100.x + (a101.x - a100.x) * dx;
+    const float c11 = a110.x + (a111.x - a110.x) * dx;
+    const float c0 = c00 + (c10 - c00) * dy;
+    const float c1 = c01 + (c11 - c01) * dy;
+    *p = c0 + (c1 - c0) * dz;
+    // round() is half away from zero: it moves up from floor when the
+    // fraction exceeds one half, or equals it for non-negative values
+    const double hx = c.xs - fx, hy = c.ys - fy, hz = c.zs - fz;
+    const bool ux = hx > 0.5 || (hx == 0.5 && c.xs >= 0.0);
+    const bool uy = hy > 0.5 || (hy == 0.5 && c.ys >= 0.0);
+    const bool uz = hz > 0.5 || (hz == 0.5 && c.zs >= 0.0);
     const float2 n0 = uy ? (ux ? a011 : a010) : (ux ? a001 : a000);
     const float2 n1 = uy ? (ux ? a111 : a110) : (ux ? a101 : a100);
     *w = uz ? n1.y : n0.y;
@@ -210,6 +223,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     const int iz = blockIdx.x;
     const int b = blockIdx.y;
     const int gx = threadIdx.x / TX, tx = threadIdx.x % TX;
+    const int sz = signed_coord<N>(iz, r);
     double R[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
@@ -223,6 +237,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
         for (int it = 0; it < ITERS; ++it) {
             const int row = it * RPI + gx;
             const bool active = row < N;
+            const int sy = signed_coord<N>(row, r);
             float2 v[EX];
             if (active) {
 #pragma unroll
@@ -232,10 +247,10 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         if (g == 0) {
-                            int z, y, x;
+                            const int x = signed_coord<N>(2 * m + h, r);
                             float p = 0.f, wv = 0.f;
-                            if (signed_triple(iz, row, 2 * m + h, N, N, N, r, &z, &y, &x))
-                                gather_pw<N>(tm, occ, R, z, y, x, &p, &wv);
+                            if (sz * sz + sy * sy + x * x <= r * r)
+                                gather_pw<N>(tm, occ, R, sz, sy, x, &p, &wv);
                             pv[h] = p;
                             w[it][e][h] = wv;
                         } else if (g == 1) {
@@ -586,8 +601,8 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         set_error("fused: invalid batch or chunk sizes");
         return kErrArg;
     }
-    if (a.rmax < 0 || a.rmax > a.n - 2) {
-        set_error("fused: invalid rmax");
+    if (a.rmax < 0 || a.rmax > a.n / 2) {
+        set_error("fused: rmax must be in [0, n/2]");
         return kErrArg;
     }
     switch (a.n) {
