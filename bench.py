@@ -53,6 +53,21 @@ def alg_bytes_per_launch(kernel, V, H, B, binary=False):
     return per_rot * B + per_launch
 
 
+def measured_traffic(workload, mode, kernel, B, binary):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (profiles/<workload>_fused_traffic.json), scaled to B rotations;
+    None when no summary covers this configuration."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_fused_traffic.json")
+    if mode not in ("fused", "auto") or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        prof = json.load(fh)
+    k = prof.get("kernels", {}).get(kernel)
+    if k is None or prof.get("binary_mask") != binary:
+        return None
+    return k["bytes_per_rotation"] * B
+
+
 def path_bytes_per_rotation(V, H):
     """76 V + 120 H: the reference GPU dataflow's bytes per rotation."""
     return 76 * V + 120 * H
@@ -84,7 +99,8 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=0,
+                    help="rotations per launch (0 = autotune batch and chunk sizes)")
     ap.add_argument("--mode", default="fused", choices=["fused", "batched", "plain", "auto"])
     ap.add_argument("--workload", default="search128")
     ap.add_argument("--plain-rotations", type=int, default=256,
@@ -118,17 +134,25 @@ def main(argv=None):
     case = make_case(w)
     R = rotations_for_angle(w.angle)
     K = R.shape[0]
-    B = 1 if args.mode == "plain" else args.batch
-    steps = args.steps if args.steps is not None else max(1, K // B)
     nz, ny, nx = case.target.shape
     V = nz * ny * nx
     H = nz * ny * (nx // 2 + 1)
 
     corr = HipCorrelator(case.target.astype(np.float64), laplace=case.laplace, device=0,
-                         mode=args.mode, batch=B)
+                         mode=args.mode, batch=max(1, args.batch))
     corr.template = case.template
     corr.mask = case.mask
     corr.rotations = R
+    tuning = None
+    if args.mode == "plain":
+        B = 1
+    elif args.batch > 0:
+        B = args.batch
+    else:
+        from powerfit_amd.autotune import autotune
+        tuning = autotune(corr, use_cache=False)
+        B = corr.batch
+    steps = args.steps if args.steps is not None else max(1, K // B)
 
     # warmup
     corr.reset()
@@ -202,9 +226,13 @@ def main(argv=None):
                                f"{w.atoms}-atom chain at {w.resolution} A, {w.angle} deg "
                                f"rotation set (K={K})",
                    "grid": [nz, ny, nx], "rotations": K, "batch": B, "mode": args.mode,
+                   "autotuned": None if tuning is None else
+                   {k: tuning[k] for k in ("batch", "chunk_z", "chunk_lcc", "trials")},
                    "laplace": w.laplace, "core_weighted": w.core_weighted},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "alg_bytes_per_launch": alg_bytes_per_launch(dom, V, H, B, binary),
+                     "traffic": measured_traffic(args.workload, args.mode, dom, B, binary)},
         "path_roofline": {"bytes_per_rotation": path_bytes_per_rotation(V, H),
                           "achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": path_gbs / HBM_PEAK_GBS},

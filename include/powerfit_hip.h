@@ -133,6 +133,10 @@ int pf_sync(pf_ctx *ctx);
 #define PF_K_COUNT 10
 /* 1 when PF_MODE_FUSED is available for this ctx's grid. */
 int pf_ctx_fused_available(pf_ctx *ctx);
+/* Rotations per block in the fused z-correlation and LCC kernels (0 = choose
+ * from the batch size so that each launch has at least 4 blocks per CU).
+ * The results do not depend on these values. */
+int pf_ctx_set_tuning(pf_ctx *ctx, int chunk_z, int chunk_lcc);
 int pf_ctx_profile(pf_ctx *ctx, int enable);
 int pf_ctx_kernel_time(pf_ctx *ctx, int which, double *ms, int64_t *launches);
 

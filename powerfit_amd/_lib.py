@@ -49,6 +49,7 @@ _SIGNATURES = {
     "pf_sync": ([_vp], _c_int),
     "pf_ctx_profile": ([_vp, _c_int], _c_int),
     "pf_ctx_fused_available": ([_vp], _c_int),
+    "pf_ctx_set_tuning": ([_vp, _c_int, _c_int], _c_int),
     "pf_ctx_kernel_time": ([_vp, _c_int, _dp, ctypes.POINTER(_c_i64)], _c_int),
 }
 

@@ -205,6 +205,11 @@ class HipCorrelator(BaseCorrelator):
         _lib.check(self._libref.pf_scan(self._ctx, first, count, self.batch,
                                         _lib.MODES[self.mode]), "pf_scan")
 
+    def set_tuning(self, chunk_z=0, chunk_lcc=0):
+        """Rotations per block in the fused z and LCC kernels (0 = automatic)."""
+        _lib.check(self._libref.pf_ctx_set_tuning(self._ctx, int(chunk_z), int(chunk_lcc)),
+                   "pf_ctx_set_tuning")
+
     @property
     def fused_available(self):
         return bool(self._libref.pf_ctx_fused_available(self._ctx))

@@ -28,6 +28,8 @@ struct pf_ctx {
     unsigned long long* d_keys = nullptr;  // running (LCC, index) keys
     uint32_t flags = 0;
     bool mask_binary = false;
+    int chunk_z = 0, chunk_lcc = 0;   // 0 = automatic
+    int num_cu = 0;
     double* d_rot = nullptr;
     int64_t nrot = 0;
     double norm = 0.0;
@@ -254,6 +256,11 @@ int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** o
     c->H = (int64_t)nz * ny * (nx / 2 + 1);
     DeviceGuard g(device);
     int s = g.ok ? kOk : kErrHip;
+    if (!s) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+        if (c->num_cu <= 0) c->num_cu = 256;
+    }
     if (!s) s = fft3d_init(&c->fft, nz, ny, nx);
     if (!s && hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
@@ -443,8 +450,15 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     fa_args.keys = c->d_keys;
     fa_args.norm = c->norm;
     fa_args.invV = invV;
-    fa_args.chunk_z = 8;
-    fa_args.chunk_lcc = 16;
+    // enough blocks: z kernel ~ ceil(H / (N * 32)) tiles per chunk, LCC kernel N planes
+    {
+        const int64_t tiles = (c->H / c->nz + 31) / 32;
+        const int64_t want = 4LL * c->num_cu;
+        const int bz = (int)std::max<int64_t>(1, std::min<int64_t>(batch, tiles * batch / want));
+        const int bl = (int)std::max<int64_t>(1, std::min<int64_t>(batch, (int64_t)c->nz * batch / want));
+        fa_args.chunk_z = c->chunk_z > 0 ? c->chunk_z : std::min(bz, 16);
+        fa_args.chunk_lcc = c->chunk_lcc > 0 ? c->chunk_lcc : std::min(bl, 255);
+    }
     for (int64_t k0 = first; k0 < first + count; k0 += batch) {
         const int nb = (int)std::min<int64_t>(batch, first + count - k0);
         fa_args.rots = c->d_rot + 9 * k0;
@@ -501,6 +515,15 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
             }));
         }
     }
+    return kOk;
+}
+
+int pf_ctx_set_tuning(pf_ctx* c, int chunk_z, int chunk_lcc) {
+    if (!c) return arg_error("ctx is null");
+    if (chunk_z < 0 || chunk_lcc < 0 || chunk_lcc > 255 || chunk_z > 4096)
+        return arg_error("chunk sizes out of range");
+    c->chunk_z = chunk_z;
+    c->chunk_lcc = chunk_lcc;
     return kOk;
 }
 

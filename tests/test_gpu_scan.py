@@ -185,3 +185,37 @@ def test_fused_batch_sizes_identical():
     outs = [run_hip(c, R, "fused", b) for b in (5, 16, 40)]
     for other in outs[1:]:
         assert np.array_equal(outs[0][0], other[0]) and np.array_equal(outs[0][1], other[1])
+
+
+def test_tuning_does_not_change_results():
+    """Per-block rotation chunks change launch shapes only."""
+    c, R, ref = case_and_oracle("cpu64", nrot=96)
+    say("scan cpu64 fused with chunk sizes (1,1) (3,5) (16,16) auto")
+    outs = []
+    for cz, cl in ((1, 1), (3, 5), (16, 16), (0, 0)):
+        corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused",
+                             batch=32)
+        corr.set_tuning(cz, cl)
+        corr.template = c.template
+        corr.mask = c.mask
+        corr.rotations = R
+        corr.scan()
+        outs.append((corr.lcc.copy(), corr.rot.copy()))
+    for other in outs[1:]:
+        assert np.array_equal(outs[0][0], other[0]) and np.array_equal(outs[0][1], other[1])
+    assert_scan_parity(*outs[0], ref)
+
+
+def test_autotune_picks_a_working_shape(tmp_path, monkeypatch):
+    from powerfit_amd.autotune import autotune
+    monkeypatch.setenv("POWERFIT_AMD_TUNE", str(tmp_path / "tune.json"))
+    c, R, ref = case_and_oracle("cpu64", nrot=96)
+    corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused")
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    t = autotune(corr, batches=(16, 32), chunks=((0, 0), (4, 4)), nrot=64)
+    assert t["batch"] in (16, 32) and t["rate"] > 0 and not t["cached"]
+    assert autotune(corr, batches=(16, 32), chunks=((0, 0), (4, 4)), nrot=64)["cached"]
+    corr.scan()
+    assert_scan_parity(corr.lcc, corr.rot, ref)
