@@ -105,6 +105,8 @@ def main(argv=None):
     ap.add_argument("--workload", default="search128")
     ap.add_argument("--plain-rotations", type=int, default=256,
                     help="rotations timed on the plain one-rotation-per-launch path")
+    ap.add_argument("--parity-rotations", type=int, default=4,
+                    help="rotations in the FP64-oracle parity sample (0 = skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 256 MiB device-to-device copy first (calibrates HBM byte "
                          "counters under a profiler)")
@@ -200,6 +202,36 @@ def main(argv=None):
         plain_value = args.plain_rotations / (time.perf_counter() - t)
         del plain
 
+    # parity on a sample: the FP64 oracle vs the same path on the same rotations
+    parity = None
+    if args.parity_rotations > 0:
+        from oracle import correlator as oc
+        best_full = np.zeros(case.target.shape, np.float32)
+        rot_full = np.zeros(case.target.shape, np.int32)
+        corr.reset()
+        run_scan(corr, K, B, max(1, K // B))
+        corr.sync()
+        _lib.check(_lib.load().pf_get_result(corr._ctx, _lib.ptr(best_full, _lib.FP),
+                                             _lib.ptr(rot_full, _lib.I32P)))
+        peak_rot = int(rot_full.reshape(-1)[np.argmax(best_full)])
+        pick = sorted(set(range(min(args.parity_rotations - 1, K))) | {peak_rot})
+        Rs = R[pick]
+        sample = HipCorrelator(case.target.astype(np.float64), laplace=case.laplace, device=0,
+                               mode=args.mode, batch=max(1, min(B, len(pick))))
+        sample.template = case.template
+        sample.mask = case.mask
+        sample.rotations = Rs
+        sample.scan()
+        ref = oc.scan(case.target.astype(np.float64), case.template, case.mask, Rs,
+                      laplace=case.laplace)
+        decisive = ref.margin() > 1e-4
+        parity = {"rotations": pick, "max_abs_lcc_err": float(np.abs(sample.lcc - ref.best).max()),
+                  "index_mismatches": int(((sample.rot != ref.rot.astype(np.int64))
+                                           & decisive).sum()),
+                  "decisive_voxels": int(decisive.sum()), "tolerance": 1e-4,
+                  "full_scan_peak_lcc": float(best_full.max())}
+        del sample
+
     cpu = None
     if args.cpu_per_worker > 0:
         nproc = min(16, os.cpu_count() or 1)
@@ -241,6 +273,7 @@ def main(argv=None):
         "plain_value": plain_value,
         "speedup_vs_plain": (value / plain_value) if plain_value else None,
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     print(json.dumps(out))
     return 0

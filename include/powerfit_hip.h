@@ -111,10 +111,22 @@ int pf_reset_result(pf_ctx *ctx);
  * ties keep the lowest index, global rotation indices.  Enqueues only. */
 int pf_scan(pf_ctx *ctx, int64_t first, int64_t count, int batch, int mode);
 
+/* pf_scan recorded once into a HIP graph (per ctx, for the last
+ * (first, count, batch, mode) used) and replayed: same launches, one graph
+ * launch.  Needs the work buffers sized beforehand (done here, outside the
+ * capture) and kernel timing off.  Enqueues only. */
+int pf_scan_graph(pf_ctx *ctx, int64_t first, int64_t count, int batch, int mode);
+
 /* Copy out best LCC (float32) and rotation index (int32), shape (nz, ny, nx)
  * — GPUCorrelator's lcc/rot (:536-537).  Synchronises the ctx stream. */
 int pf_get_result(pf_ctx *ctx, float *best_lcc, int32_t *best_rot);
 int pf_sync(pf_ctx *ctx);
+/* Non-blocking: 1 when all work enqueued on the ctx stream has finished,
+ * 0 when some is still running (hipStreamQuery). */
+int pf_ctx_query(pf_ctx *ctx);
+/* Device bytes a ctx for an (nz, ny, nx) grid allocates when scanning with
+ * `batch` rotations per launch in `mode` (for admission control). */
+int64_t pf_ctx_bytes(int nz, int ny, int nx, int batch, int mode);
 
 /* Kernel timing of subsequent pf_scan calls with HIP events on the ctx
  * stream (0 = off).  pf_ctx_kernel_time returns, for kernel class `which`

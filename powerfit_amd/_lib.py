@@ -45,11 +45,14 @@ _SIGNATURES = {
     "pf_set_rotations": ([_vp, _dp, _c_i64], _c_int),
     "pf_reset_result": ([_vp], _c_int),
     "pf_scan": ([_vp, _c_i64, _c_i64, _c_int, _c_int], _c_int),
+    "pf_scan_graph": ([_vp, _c_i64, _c_i64, _c_int, _c_int], _c_int),
     "pf_get_result": ([_vp, _fp, _i32p], _c_int),
     "pf_sync": ([_vp], _c_int),
     "pf_ctx_profile": ([_vp, _c_int], _c_int),
     "pf_ctx_fused_available": ([_vp], _c_int),
     "pf_ctx_set_tuning": ([_vp, _c_int, _c_int], _c_int),
+    "pf_ctx_query": ([_vp], _c_int),
+    "pf_ctx_bytes": ([_c_int, _c_int, _c_int, _c_int, _c_int], _c_i64),
     "pf_ctx_kernel_time": ([_vp, _c_int, _dp, ctypes.POINTER(_c_i64)], _c_int),
 }
 

@@ -214,6 +214,26 @@ class HipCorrelator(BaseCorrelator):
     def fused_available(self):
         return bool(self._libref.pf_ctx_fused_available(self._ctx))
 
+    def enqueue_scan_graph(self, first, count):
+        """Enqueue [first, first+count) as one replay of a captured HIP graph."""
+        _lib.check(self._libref.pf_scan_graph(self._ctx, first, count, self.batch,
+                                              _lib.MODES[self.mode]), "pf_scan_graph")
+
+    def done(self):
+        """True when all enqueued work on this correlator's stream has finished."""
+        rc = self._libref.pf_ctx_query(self._ctx)
+        if rc < 0:
+            _lib.check(rc, "pf_ctx_query")
+        return rc == 1
+
+    def result(self):
+        """Copy out (lcc float32, rot int32) of the work enqueued so far (syncs)."""
+        self._lcc = np.zeros(self._target.shape, dtype=np.float32)
+        self._rot = np.zeros(self._target.shape, dtype=np.int32)
+        _lib.check(self._libref.pf_get_result(self._ctx, _lib.ptr(self._lcc, _lib.FP),
+                                              _lib.ptr(self._rot, _lib.I32P)), "pf_get_result")
+        return self._lcc, self._rot
+
     def reset(self):
         _lib.check(self._libref.pf_reset_result(self._ctx), "pf_reset_result")
 

@@ -29,6 +29,11 @@ struct pf_ctx {
     uint32_t flags = 0;
     bool mask_binary = false;
     int chunk_z = 0, chunk_lcc = 0;   // 0 = automatic
+    // captured scan (pf_scan_graph)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t g_first = -1, g_count = -1;
+    int g_batch = -1, g_mode = -1, g_cz = -1, g_cl = -1;
     int num_cu = 0;
     double* d_rot = nullptr;
     int64_t nrot = 0;
@@ -296,6 +301,8 @@ int pf_ctx_destroy(pf_ctx* c) {
             (void)hipEventDestroy(r.b);
         }
         for (auto e : c->pool) (void)hipEventDestroy(e);
+        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+        if (c->graph) (void)hipGraphDestroy(c->graph);
         dev_free(c->d_template);
         dev_free(c->d_mask);
         dev_free(c->d_F1);
@@ -518,12 +525,80 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     return kOk;
 }
 
+int pf_ctx_query(pf_ctx* c) {
+    if (!c) return arg_error("ctx is null");
+    PF_DEVICE_GUARD(c);
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    set_error("hipStreamQuery: %s", hipGetErrorString(e));
+    return kErrHip;
+}
+
+int64_t pf_ctx_bytes(int nz, int ny, int nx, int batch, int mode) {
+    if (nz < 1 || ny < 1 || nx < 1 || batch < 1) return arg_error("invalid shape or batch");
+    const int64_t V = (int64_t)nz * ny * nx, H = (int64_t)nz * ny * (nx / 2 + 1);
+    if (mode == PF_MODE_PLAIN) batch = 1;
+    const bool fused = (mode == PF_MODE_FUSED || mode == PF_MODE_AUTO) &&
+                       fused_supported(nz, ny, nx);
+    // template, mask (f32), packed pair, keys, lcc mask, two target spectra
+    int64_t bytes = V * (4 + 4 + 8 + 8 + 1) + 2 * H * 8;
+    bytes += (int64_t)3 * batch * H * 8;             // complex work
+    if (!fused) bytes += (int64_t)3 * batch * V * 4;  // real work of the unfused stages
+    return bytes;
+}
+
 int pf_ctx_set_tuning(pf_ctx* c, int chunk_z, int chunk_lcc) {
     if (!c) return arg_error("ctx is null");
     if (chunk_z < 0 || chunk_lcc < 0 || chunk_lcc > 255 || chunk_z > 4096)
         return arg_error("chunk sizes out of range");
     c->chunk_z = chunk_z;
     c->chunk_lcc = chunk_lcc;
+    return kOk;
+}
+
+int pf_scan_graph(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
+    if (!c) return arg_error("ctx is null");
+    if (c->profile) {
+        set_error("pf_scan_graph: disable kernel timing first");
+        return kErrState;
+    }
+    PF_DEVICE_GUARD(c);
+    const bool same = c->graph_exec && c->g_first == first && c->g_count == count &&
+                      c->g_batch == batch && c->g_mode == mode && c->g_cz == c->chunk_z &&
+                      c->g_cl == c->chunk_lcc;
+    if (!same) {
+        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+        if (c->graph) (void)hipGraphDestroy(c->graph);
+        c->graph_exec = nullptr;
+        c->graph = nullptr;
+        // validate and size the work buffers outside the capture: a zero-length scan
+        PF_CHECK_STATUS(pf_scan(c, first, 0, batch, mode));
+        PF_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int s = pf_scan(c, first, count, batch, mode);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (s) {
+            if (g) (void)hipGraphDestroy(g);
+            return s;
+        }
+        if (e != hipSuccess) {
+            set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+            return kErrHip;
+        }
+        c->graph = g;
+        PF_HIP_CHECK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->g_first = first;
+        c->g_count = count;
+        c->g_batch = batch;
+        c->g_mode = mode;
+        c->g_cz = c->chunk_z;
+        c->g_cl = c->chunk_lcc;
+    }
+    PF_HIP_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
     return kOk;
 }
 

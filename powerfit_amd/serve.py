@@ -1,0 +1,217 @@
+"""Continuous-batching request loop for LCC searches on one GPU.
+
+Each admitted fit gets its own correlator context: its own HIP stream, its
+target spectra resident in HBM for the whole fit.  Requests are admitted as
+slots and memory free up — a fit is admitted when the bytes its context will
+allocate (``pf_ctx_bytes``) fit in the remaining HBM budget and a slot is
+free — so fits of mixed grid sizes share the device.  The loop interleaves
+the active fits batch by batch (each keeps at most ``depth`` batches queued on
+its stream), so work from several fits overlaps on the device and a request
+arriving mid-way starts as soon as it is admitted instead of waiting for the
+queue to drain.  A fit's result is read back when its stream is idle; its
+context is then released and the memory returned to the budget.
+
+Each fit's result is computed exactly as in a standalone search: its
+rotations go through the same kernels, and the per-voxel merge is an
+order-independent max, so a result is identical alone and in the mix.
+
+    python -m powerfit_amd.serve --requests 12 --rate 4
+
+prints one JSON line with throughput and p50/p99 latency.
+"""
+import argparse
+import json
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .correlator import HipCorrelator
+
+
+@dataclass
+class FitRequest:
+    """One fit: map, template, mask, rotation set, options, arrival time (s)."""
+    rid: int
+    target: np.ndarray
+    template: np.ndarray
+    mask: np.ndarray
+    rotations: np.ndarray
+    laplace: bool = False
+    arrival: float = 0.0
+
+
+@dataclass
+class FitResult:
+    rid: int
+    lcc: np.ndarray
+    rot: np.ndarray
+    arrival: float
+    admitted: float
+    finished: float
+    rotations: int
+    shape: tuple
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+@dataclass
+class _Active:
+    req: FitRequest
+    corr: HipCorrelator
+    bytes: int
+    admitted: float
+    next_rot: int = 0
+    queued: list = field(default_factory=list)
+
+
+def ctx_bytes(shape, batch, mode="auto"):
+    """Device bytes a context for `shape` allocates (admission control)."""
+    nz, ny, nx = shape
+    b = _lib.load().pf_ctx_bytes(nz, ny, nx, int(batch), _lib.MODES[mode])
+    if b < 0:
+        _lib.check(int(b), "pf_ctx_bytes")
+    return int(b)
+
+
+class FitServer:
+    """Admit, interleave and complete LCC searches on one device."""
+
+    def __init__(self, device=0, hbm_budget=64 << 30, max_active=8, batch=64, depth=2,
+                 mode="auto"):
+        self.device = device
+        self.hbm_budget = int(hbm_budget)
+        self.max_active = max_active
+        self.batch = batch
+        self.depth = depth
+        self.mode = mode
+        self.used = 0
+        self.active = []
+        self.waiting = []
+        self.results = {}
+
+    def _admit(self, now):
+        while self.waiting and len(self.active) < self.max_active:
+            req = self.waiting[0]
+            need = ctx_bytes(req.target.shape, self.batch, self.mode)
+            if need > self.hbm_budget:
+                raise MemoryError(f"request {req.rid} needs {need} bytes, over the budget")
+            if self.used + need > self.hbm_budget:
+                break
+            self.waiting.pop(0)
+            corr = HipCorrelator(np.asarray(req.target, np.float64), laplace=req.laplace,
+                                 device=self.device, mode=self.mode, batch=self.batch)
+            corr.template = req.template
+            corr.mask = req.mask
+            corr.rotations = req.rotations
+            corr.reset()
+            self.used += need
+            self.active.append(_Active(req, corr, need, now))
+
+    def _pump(self):
+        """Queue one more batch on every active fit whose stream is not full."""
+        for a in self.active:
+            K = a.req.rotations.shape[0]
+            if a.next_rot >= K:
+                continue
+            if len(a.queued) >= self.depth:
+                if not a.corr.done():
+                    continue
+                a.queued.clear()
+            n = min(self.batch, K - a.next_rot)
+            a.corr.enqueue_scan(a.next_rot, n)
+            a.next_rot += n
+            a.queued.append(n)
+
+    def _complete(self, now_fn):
+        still = []
+        for a in self.active:
+            if a.next_rot >= a.req.rotations.shape[0] and a.corr.done():
+                lcc, rot = a.corr.result()
+                self.results[a.req.rid] = FitResult(
+                    a.req.rid, lcc.copy(), rot.copy(), a.req.arrival, a.admitted, now_fn(),
+                    a.req.rotations.shape[0], a.req.target.shape)
+                self.used -= a.bytes
+                del a.corr
+            else:
+                still.append(a)
+        self.active = still
+
+    def run(self, requests, realtime=True):
+        """Serve `requests` (arrival times in seconds from the start when
+        realtime, else all at once).  Returns {rid: FitResult}."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        t0 = time.perf_counter()
+
+        def now():
+            return time.perf_counter() - t0
+
+        if not realtime:
+            for r in pending:
+                r.arrival = 0.0
+        while pending or self.waiting or self.active:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                self.waiting.append(pending.pop(0))
+            self._admit(t)
+            self._pump()
+            self._complete(now)
+            if not self.active and not self.waiting and pending:
+                time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+            elif self.active and all(a.next_rot >= a.req.rotations.shape[0] or
+                                     len(a.queued) >= self.depth for a in self.active):
+                time.sleep(0.0002)
+        return self.results
+
+
+def make_stream(n, rate, seed=5003, sizes=None, angles=(10.0, 20.0)):
+    """Poisson stream of synthetic fit requests (config: 500-20,000-atom chains
+    into 48^3-160^3 maps at 4-12 A, 10 or 20 degree rotation sets)."""
+    from .rotations import rotations_for_angle
+    from .synth import Workload, make_case
+    rng = np.random.default_rng(seed)
+    sizes = sizes or [48, 64, 72, 80, 96, 120, 128, 144, 160]
+    t = 0.0
+    reqs = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        N = int(rng.choice(sizes))
+        res = float(rng.uniform(4.0, 12.0))
+        atoms = int(rng.integers(500, 20001))
+        angle = float(rng.choice(angles))
+        w = Workload(f"req{i}", N, res / 4.0, atoms, res, angle, seed=seed * 10 + i * 7)
+        c = make_case(w)
+        reqs.append(FitRequest(i, c.target, c.template, c.mask, rotations_for_angle(angle),
+                               c.laplace, t))
+    return reqs
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="serve a Poisson stream of LCC fits")
+    ap.add_argument("--requests", type=int, default=12)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--seed", type=int, default=5003)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--max-active", type=int, default=8)
+    ap.add_argument("--budget-gib", type=float, default=64.0)
+    args = ap.parse_args(argv)
+    reqs = make_stream(args.requests, args.rate, args.seed)
+    server = FitServer(hbm_budget=int(args.budget_gib * (1 << 30)), max_active=args.max_active,
+                       batch=args.batch)
+    t = time.perf_counter()
+    res = server.run(reqs)
+    wall = time.perf_counter() - t
+    lat = np.array([r.latency for r in res.values()])
+    rots = sum(r.rotations for r in res.values())
+    print(json.dumps({
+        "requests": len(res), "wall_s": wall, "rotations": rots,
+        "rotations_per_s": rots / wall, "requests_per_s": len(res) / wall,
+        "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+        "sizes": sorted({r.shape[0] for r in res.values()}), "arrival_rate": args.rate}))
+
+
+if __name__ == "__main__":
+    main()

@@ -30,3 +30,12 @@ def test_argument_errors_need_no_device():
     assert rc == -1 and b"radius" in lib.pf_last_error()
     assert lib.pf_ctx_create(0, 8, 8, 8, 0, None) == -1
     assert lib.pf_scan(None, 0, 1, 1, 0) == -1
+
+
+def test_ctx_bytes_model():
+    lib = _lib.load()
+    fused = lib.pf_ctx_bytes(128, 128, 128, 64, _lib.MODES["fused"])
+    unfused = lib.pf_ctx_bytes(128, 128, 128, 64, _lib.MODES["batched"])
+    plain = lib.pf_ctx_bytes(128, 128, 128, 64, _lib.MODES["plain"])
+    assert 0 < plain < fused < unfused
+    assert lib.pf_ctx_bytes(0, 1, 1, 1, 0) == -1

@@ -219,3 +219,72 @@ def test_autotune_picks_a_working_shape(tmp_path, monkeypatch):
     assert autotune(corr, batches=(16, 32), chunks=((0, 0), (4, 4)), nrot=64)["cached"]
     corr.scan()
     assert_scan_parity(corr.lcc, corr.rot, ref)
+
+
+@pytest.mark.parametrize("mode", ["fused", "batched"])
+def test_graph_replays_match_eager(mode):
+    """A captured scan replayed several times, with eager work and syncs in
+    between, gives the eager result every time."""
+    c, R, ref = case_and_oracle("cpu64", nrot=96)
+    say(f"graph capture cpu64 mode={mode}")
+    corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode=mode, batch=16)
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    corr.scan()
+    eager = (corr.lcc.copy(), corr.rot.copy())
+    for rep in range(3):
+        corr.reset()
+        corr.enqueue_scan_graph(0, len(R))
+        lcc, rot = corr.result()
+        assert np.array_equal(lcc, eager[0]) and np.array_equal(rot, eager[1]), rep
+        corr.scan()                      # eager work between replays
+        assert np.array_equal(corr.lcc, eager[0])
+    assert_scan_parity(*eager, ref)
+
+
+def test_serving_results_equal_standalone():
+    """Fits admitted together and interleaved on the device give exactly the
+    result each gives alone."""
+    from powerfit_amd.serve import FitRequest, FitServer
+    say("serving: three fits alone and in the mix")
+    reqs = []
+    for i, (name, nrot) in enumerate([("cpu64", 40), ("t_odd", 40), ("lc96", 24)]):
+        w = SMALL.get(name) or WORKLOADS[name]
+        cs = make_case(w)
+        R = rotations_for_angle(w.angle)[:nrot]
+        reqs.append(FitRequest(i, cs.target, cs.template, cs.mask, R, cs.laplace, 0.0))
+    alone = {}
+    for r in reqs:
+        corr = HipCorrelator(r.target.astype(np.float64), laplace=r.laplace, batch=8)
+        corr.template = r.template
+        corr.mask = r.mask
+        corr.rotations = r.rotations
+        corr.scan()
+        alone[r.rid] = (corr.lcc.copy(), corr.rot.copy())
+    server = FitServer(max_active=3, batch=8, depth=2)
+    res = server.run(reqs, realtime=False)
+    assert sorted(res) == [0, 1, 2] and server.used == 0
+    for rid, (lcc, rot) in alone.items():
+        assert np.array_equal(res[rid].lcc, lcc) and np.array_equal(res[rid].rot, rot), rid
+
+
+def test_serving_admission_respects_budget():
+    from powerfit_amd.serve import FitRequest, FitServer, ctx_bytes
+    w = WORKLOADS["cpu64"]
+    cs = make_case(w)
+    R = rotations_for_angle(w.angle)[:16]
+    need = ctx_bytes(cs.target.shape, 8)
+    reqs = [FitRequest(i, cs.target, cs.template, cs.mask, R, False, 0.0) for i in range(3)]
+    server = FitServer(hbm_budget=int(need * 1.5), max_active=8, batch=8)
+    peak = []
+    orig = server._pump
+
+    def pump():
+        peak.append(len(server.active))
+        orig()
+    server._pump = pump
+    res = server.run(reqs, realtime=False)
+    assert len(res) == 3 and max(peak) == 1     # budget admits one fit at a time
+    for r in res.values():
+        assert np.array_equal(r.lcc, res[0].lcc)
