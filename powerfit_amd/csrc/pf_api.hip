@@ -2,6 +2,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "../../include/powerfit_hip.h"
@@ -29,6 +30,7 @@ struct pf_ctx {
     uint32_t flags = 0;
     bool mask_binary = false;
     int chunk_z = 0, chunk_lcc = 0;   // 0 = automatic
+    int support_zr = 0;               // |v| > support_zr: rotated template and mask are zero
     // captured scan (pf_scan_graph)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -392,6 +394,23 @@ int pf_set_template_mask(pf_ctx* c, const double* tmpl, const double* mask, doub
     if (!s) {
         bool binary = true;
         for (int64_t i = 0; i < c->V && binary; ++i) binary = (mask[i] == 0.0 || mask[i] == 1.0);
+        // support radius of template and mask about voxel 0 (periodic signed
+        // coordinates); a rotated voxel v samples corners within sqrt(3) of
+        // R^T v, |R^T v| = |v|, so |v| > R + 2 reads only zeros
+        int64_t r2max = 0;
+        for (int z = 0; z < c->nz; ++z) {
+            const int64_t sz = z <= c->nz / 2 ? z : z - c->nz;
+            for (int y = 0; y < c->ny; ++y) {
+                const int64_t sy = y <= c->ny / 2 ? y : y - c->ny;
+                const int64_t base = ((int64_t)z * c->ny + y) * c->nx;
+                for (int x = 0; x < c->nx; ++x) {
+                    if (tmpl[base + x] == 0.0 && mask[base + x] == 0.0) continue;
+                    const int64_t sx = x <= c->nx / 2 ? x : x - c->nx;
+                    r2max = std::max(r2max, sz * sz + sy * sy + sx * sx);
+                }
+            }
+        }
+        c->support_zr = (int)std::ceil(std::sqrt((double)r2max)) + 2;
         c->mask_binary = binary && !(c->flags & PF_FLAG_NO_BINARY_MASK);
         c->norm = norm_factor;
         c->rmax = rmax;
@@ -450,6 +469,8 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     fa_args.tm = c->d_tm;
     fa_args.occ = c->d_occ;
     fa_args.rmax = c->rmax;
+    fa_args.zr = c->support_zr;
+    fa_args.skip_planes = fa && fb;
     fa_args.work = c->d_cplx;
     fa_args.F1 = c->d_F1;
     fa_args.F2 = c->d_F2;

@@ -49,8 +49,10 @@ extern "C" int pf_host_plan(int n, int* radix, int max) {
 // (N, T) pairs the fused kernels use.
 #include "pf_group_fft.h"
 
+// ex_out: the exchange buffer after the last stage (the natural-order result
+// group_fft_ex leaves there) instead of the registers.
 template <int N, int T>
-static int group_fft_host(int dir, const float* in, float* out) {
+static int group_fft_host(int dir, const float* in, float* out, bool ex_out) {
     constexpr int E = N / T;
     std::vector<float2> tw(N), ex(N);
     for (int m = 0; m < N; ++m) {
@@ -65,22 +67,32 @@ static int group_fft_host(int dir, const float* in, float* out) {
     else pf::GroupStage<N, T, 1, 0>::emulate(v, ex.data(), tw.data());
     for (int t = 0; t < T; ++t)
         for (int e = 0; e < E; ++e) {
-            out[2 * (t + T * e)] = v[t][e].x;
-            out[2 * (t + T * e) + 1] = v[t][e].y;
+            const float2 r = ex_out ? ex[t + T * e] : v[t][e];
+            out[2 * (t + T * e)] = r.x;
+            out[2 * (t + T * e) + 1] = r.y;
         }
     return pf::GroupStage<N, T, -1, 0>::P.ns;
 }
 
 #define PF_GROUP_CASE(N, T) \
-    if (n == N && t == T) return group_fft_host<N, T>(dir, in, out);
+    if (n == N && t == T) return group_fft_host<N, T>(dir, in, out, ex_out);
 
-extern "C" int pf_host_group_fft(int n, int t, int dir, const float* in, float* out) {
+static int group_fft_any(int n, int t, int dir, const float* in, float* out, bool ex_out) {
     PF_GROUP_CASE(16, 1) PF_GROUP_CASE(32, 4) PF_GROUP_CASE(32, 2)
     PF_GROUP_CASE(48, 4) PF_GROUP_CASE(64, 8) PF_GROUP_CASE(64, 4)
     PF_GROUP_CASE(96, 4) PF_GROUP_CASE(96, 8) PF_GROUP_CASE(128, 8)
     PF_GROUP_CASE(128, 16) PF_GROUP_CASE(160, 8) PF_GROUP_CASE(192, 8)
     PF_GROUP_CASE(80, 4) PF_GROUP_CASE(40, 4) PF_GROUP_CASE(56, 4) PF_GROUP_CASE(100, 5)
+    PF_GROUP_CASE(24, 2) PF_GROUP_CASE(36, 3) PF_GROUP_CASE(72, 6) PF_GROUP_CASE(80, 8)
     return -1;
+}
+
+extern "C" int pf_host_group_fft(int n, int t, int dir, const float* in, float* out) {
+    return group_fft_any(n, t, dir, in, out, false);
+}
+
+extern "C" int pf_host_group_fft_ex(int n, int t, int dir, const float* in, float* out) {
+    return group_fft_any(n, t, dir, in, out, true);
 }
 
 extern "C" int pf_host_pick_T(int n, int emax, int tmax) { return pf::pick_T(n, emax, tmax); }

@@ -22,6 +22,9 @@
 // through the y transform packed as one complex column X0 + i XM and are
 // separated with Hermitian symmetry on the way out (and packed on the way in
 // for the inverse).
+#include <algorithm>
+#include <type_traits>
+
 #include "pf_group_fft.h"
 #include "pf_internal.h"
 #include "pf_rotate.h"
@@ -66,11 +69,10 @@ __device__ __forceinline__ void stage_twiddles(float2* dst, const float2* __rest
 // touches consecutive words) and stages use workgroup barriers.
 // PACK_IN: the plane holds the M+1 unpacked columns (DC at 0, Nyquist at M);
 // column 0 is transformed as DC + i Nyquist.
-template <int N, int NTHR, int DIR, bool WAVE, bool PACK_IN = false>
+template <int N, int NTHR, int DIR, bool WAVE, int EMAX, bool PACK_IN = false>
 __device__ __forceinline__ void plane_columns(float2* plane, const float2* __restrict__ twN) {
     using G = Geo<N>;
-    // enough threads per column that the M columns occupy the whole block
-    constexpr int TY = pick_T_min(N, 16, (NTHR + G::M - 1) / G::M, 32);
+    constexpr int TY = pick_T_cols(N, G::M, NTHR, EMAX);
     constexpr int EY = N / TY;
     constexpr int CPI = NTHR / TY;
     constexpr int CITERS = (G::M + CPI - 1) / CPI;
@@ -197,21 +199,27 @@ __device__ __forceinline__ void gather_pw(const float2* __restrict__ tm, const u
 }
 
 // ---------------------------------------------------------------------------
-template <int N, int NTHR, bool BIN>
+template <int N, int NTHR, bool BIN, int EMAX>
 __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ tm,
                                                     const double* __restrict__ rots, int r,
+                                                    int zr, int zlo,
                                                     float2* __restrict__ out,
                                                     const float2* __restrict__ twM,
                                                     const float2* __restrict__ twN,
                                                     const uint32_t* __restrict__ occ_g) {
     using G = Geo<N>;
     constexpr int M = G::M;
-    constexpr int TX = pick_T(M, 16, 16);
+    constexpr int TX = pick_T(M, EMAX, 16);
     constexpr int EX = M / TX;
     constexpr int RPI = NTHR / TX;
     constexpr int ITERS = (N + RPI - 1) / RPI;
     constexpr int OCCW = (G::NB * G::NB * G::NB + 31) / 32;
+    constexpr int TY = pick_T_cols(N, M, NTHR, EMAX);
+    constexpr int EY = N / TY;
+    constexpr int CPI = NTHR / TY;
+    constexpr bool WROWS = kWaveRows && (64 % TX == 0);
     static_assert(TX > 0 && NTHR % TX == 0, "row plan");
+    static_assert(TY > 0 && NTHR % TY == 0 && CPI >= M, "one column pass");
     __shared__ float2 plane[N * G::S];
     __shared__ uint32_t occ[OCCW];
     __shared__ float2 stwM[M], stwN[N];
@@ -220,9 +228,13 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     stage_twiddles<N, NTHR>(stwN, twN);
     __syncthreads();
 
-    const int iz = blockIdx.x;
+    // planes zlo, zlo + 1, ... (mod N); rows outside the support disc are zero
+    const int iz = (blockIdx.x + zlo + N) % N;
     const int b = blockIdx.y;
     const int gx = threadIdx.x / TX, tx = threadIdx.x % TX;
+    // column pass: lanes run across columns (conflict-free LDS)
+    const int cg = threadIdx.x % CPI, ty = threadIdx.x / CPI;
+    const bool cact = cg < M;
     const int sz = signed_coord<N>(iz, r);
     double R[9];
 #pragma unroll
@@ -230,14 +242,22 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     float w[ITERS][EX][2];
     float2* dst = out + ((int64_t)b * 3) * G::H + (int64_t)iz * G::PLANE;
 
-    // Row pass for grid g (0: P, 1: W, 2: W^2): fill v, x FFT, post-process.
+    // Grid g (0: P, 1: W, 2: W^2): gather the rows and run their half-length x
+    // FFT, then the column pass (R2C post-processing, y FFT) and the store.
 #pragma unroll
     for (int g = 0; g < (BIN ? 2 : 3); ++g) {
 #pragma unroll
         for (int it = 0; it < ITERS; ++it) {
             const int row = it * RPI + gx;
-            const bool active = row < N;
             const int sy = signed_coord<N>(row, r);
+            const bool in_plane = row < N;
+            // rows whose every voxel lies outside the support are exactly zero
+            const bool active = in_plane && sz * sz + sy * sy <= zr * zr;
+            if (in_plane && !active) {
+                float2* rowz = plane + row * G::S;
+#pragma unroll
+                for (int e = 0; e < EX; ++e) rowz[tx + TX * e] = c2(0.f, 0.f);
+            }
             float2 v[EX];
             if (active) {
 #pragma unroll
@@ -263,38 +283,35 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                 }
             }
             float2* rowp = plane + row * G::S;
-            group_fft<M, TX, -1, kWaveRows>(v, active ? rowp : plane, 1, tx, stwM, active);
-            // R2C post-processing: X[k] = E[k] + w^k O[k]; X0, XM packed in column 0
-            if (active) {
+            // row FFT; its result stays in the row (natural order)
+            group_fft_ex<M, TX, -1, WROWS>(v, active ? rowp : plane, 1, tx, stwM, active);
+        }
+        __syncthreads();
+        {
+            // column pass: R2C post-processing on the way in,
+            // X[k] = E[k] + w^k O[k] from z[k] and conj(z[M - k]) of each row,
+            // X0 and XM packed in column 0; then the y FFT, left in the plane
+            float2 v[EY];
+            if (cact) {
 #pragma unroll
-                for (int e = 0; e < EX; ++e) rowp[tx + TX * e] = v[e];
-            }
-            __syncthreads();
-            float2 X[EX];
-            if (active) {
-#pragma unroll
-                for (int e = 0; e < EX; ++e) {
-                    const int k = tx + TX * e;
-                    const float2 zk = rowp[k];
-                    if (k == 0) {
-                        X[e] = c2(zk.x + zk.y, zk.x - zk.y);
+                for (int e = 0; e < EY; ++e) {
+                    const float2* rowp = plane + (ty + TY * e) * G::S;
+                    const float2 zk = rowp[cg];
+                    if (cg == 0) {
+                        v[e] = c2(zk.x + zk.y, zk.x - zk.y);
                     } else {
-                        const float2 zm = cconj(rowp[M - k]);
+                        const float2 zm = cconj(rowp[M - cg]);
                         const float2 Ev = cscale(cadd(zk, zm), 0.5f);
                         const float2 Od = mul_si<-1>(cscale(csub(zk, zm), 0.5f));
                         // exp(-2 pi i k / N) = twN[k]
-                        X[e] = cadd(Ev, cmul(stwN[k], Od));
+                        v[e] = cadd(Ev, cmul(stwN[cg], Od));
                     }
                 }
             }
-            __syncthreads();
-            if (active) {
-#pragma unroll
-                for (int e = 0; e < EX; ++e) rowp[tx + TX * e] = X[e];
-            }
+            __syncthreads();   // mirror reads are done before the FFT overwrites columns
+            group_fft_ex<N, TY, -1, kWaveCols>(v, plane + (cact ? cg : 0), G::S, ty, stwN, cact);
         }
         __syncthreads();
-        plane_columns<N, NTHR, -1, kWaveCols>(plane, stwN);
         store_plane<N, NTHR>(plane, dst + (int64_t)g * G::H);
         __syncthreads();
     }
@@ -304,7 +321,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
 template <int N, int TQ, int EMAX, bool BIN>
 __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
     float2* __restrict__ X, const float2* __restrict__ F1, const float2* __restrict__ F2,
-    int nrot, int chunk, const float2* __restrict__ twN) {
+    int nrot, int chunk, int zr, const float2* __restrict__ twN) {
     using G = Geo<N>;
     constexpr int TZ = pick_T(N, EMAX, 32);
     constexpr int EZ = N / TZ;
@@ -334,7 +351,12 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
             float2 v[EZ];
             if (active) {
 #pragma unroll
-                for (int e = 0; e < EZ; ++e) v[e] = col[(int64_t)(t + TZ * e) * G::PLANE];
+                for (int e = 0; e < EZ; ++e) {
+                    // planes beyond the support were not written: they are zero
+                    const int z = t + TZ * e;
+                    const int sz = z <= N / 2 ? z : z - N;
+                    v[e] = (sz >= -zr && sz <= zr) ? col[(int64_t)z * G::PLANE] : c2(0.f, 0.f);
+                }
             }
             group_fft<N, TZ, -1>(v, exc, TQ, t, stw, active);
             if (BIN && g == 1) {
@@ -367,7 +389,7 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
 }
 
 // ---------------------------------------------------------------------------
-template <int N, int NTHR>
+template <int N, int NTHR, int EMAX>
 __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X,
                                                     const uint8_t* __restrict__ lmask,
                                                     unsigned long long* __restrict__ keys,
@@ -377,13 +399,19 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
                                                     const float2* __restrict__ twN) {
     using G = Geo<N>;
     constexpr int M = G::M;
-    constexpr int TX = pick_T(M, 16, 16);
+    constexpr int TX = pick_T(M, EMAX, 16);
     constexpr int EX = M / TX;
     constexpr int RPI = NTHR / TX;
     constexpr int ITERS = (N + RPI - 1) / RPI;
-    constexpr int PER = (G::PLANE + NTHR - 1) / NTHR;   // plane elements per thread
-    static_assert(ITERS * EX * 2 <= 32, "owned voxels per thread must fit a 32-bit mask");
+    constexpr int TY = pick_T_cols(N, M, NTHR, EMAX);
+    constexpr int EY = N / TY;
+    constexpr int CPI = NTHR / TY;
+    constexpr bool WROWS = kWaveRows && (64 % TX == 0);
+    static_assert(TY > 0 && NTHR % TY == 0 && CPI >= M, "one column pass");
+    static_assert(NTHR >= N, "one Nyquist element per thread");
+    static_assert(ITERS * EX * 2 <= 64, "owned voxels per thread must fit a 64-bit mask");
     __shared__ float2 plane[N * G::S];
+    __shared__ float2 colM[N];
     __shared__ float2 stwM[M], stwN[N];
     stage_twiddles<M, NTHR>(stwM, twM);
     stage_twiddles<N, NTHR>(stwN, twN);
@@ -392,10 +420,14 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
     const int b0 = blockIdx.y * chunk;
     const int b1 = min(nrot, b0 + chunk);
     const int gx = threadIdx.x / TX, tx = threadIdx.x % TX;
+    // column pass: lanes run across columns (coalesced loads, conflict-free LDS)
+    const int cg = threadIdx.x % CPI, ty = threadIdx.x / CPI;
+    const bool cact = cg < M;
     const float invVf = (float)invV;
 
     // owned voxels: (row = it*RPI + gx, x = 2 (tx + TX e) + h), bit 2(it*EX+e)+h
-    uint32_t mbits = 0;
+    using MaskT = typename std::conditional<(ITERS * EX * 2 <= 32), uint32_t, uint64_t>::type;
+    MaskT mbits = 0;
     float isd[ITERS][EX][2];
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
@@ -406,54 +438,51 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
             for (int h = 0; h < 2; ++h)
                 if (row < N) {
                     const int64_t vox = ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
-                    if (lmask[vox]) mbits |= 1u << (2 * (it * EX + e) + h);
+                    if (lmask[vox]) mbits |= MaskT(1) << (2 * (it * EX + e) + h);
                 }
     }
 
-    // Planes are visited in the order (b, B), (b, A), (b, G), (b + 1, B), ...;
-    // the next plane's global loads are issued into registers before the
-    // current one is transformed, so their latency overlaps the FFTs.
+    // Planes are visited in the order (b, B), (b, A), (b, G), (b + 1, B), ...
+    // Each plane goes from global memory straight into the column registers
+    // (column cg, rows ty + TY e; the Nyquist column kx = M, one element per
+    // thread, through a small LDS buffer); the next plane's loads are issued
+    // before the current one is transformed, so their latency overlaps the FFTs.
     const int nplanes = 3 * (b1 - b0);
     auto src_of = [&](int p) {
         const int b = b0 + p / 3, g = 2 - p % 3;
         return X + ((int64_t)b * 3 + g) * G::H + (int64_t)iz * G::PLANE;
     };
-    float2 pf[PER];
-    if (nplanes > 0) {
-        const float2* src = src_of(0);
+    float2 pf[EY];
+    float2 pm = c2(0.f, 0.f);
+    auto prefetch = [&](const float2* src) {
+        if (cact) {
 #pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int f = threadIdx.x + NTHR * i;
-            if (f < G::PLANE) pf[i] = src[f];
+            for (int e = 0; e < EY; ++e) pf[e] = src[(ty + TY * e) * G::NXH + cg];
         }
-    }
+        if (threadIdx.x < N) pm = src[threadIdx.x * G::NXH + M];
+    };
+    if (nplanes > 0) prefetch(src_of(0));
     for (int p = 0; p < nplanes; ++p) {
         const int b = b0 + p / 3, g = 2 - p % 3;
         __syncthreads();   // previous plane's row phase is done with the LDS
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int f = threadIdx.x + NTHR * i;
-            if (f < G::PLANE) {
-                const int ky = f / G::NXH, kx = f - ky * G::NXH;
-                plane[ky * G::S + kx] = pf[i];
-            }
-        }
-        if (p + 1 < nplanes) {
-            const float2* src = src_of(p + 1);
-#pragma unroll
-            for (int i = 0; i < PER; ++i) {
-                const int f = threadIdx.x + NTHR * i;
-                if (f < G::PLANE) pf[i] = src[f];
-            }
-        }
+        if (threadIdx.x < N) colM[threadIdx.x] = pm;
         __syncthreads();
-        plane_columns<N, NTHR, 1, kWaveCols, true>(plane, stwN);
+        // y inverse of the M packed columns: column 0 carries X0 + i XM
+        float2 v[EY];
+#pragma unroll
+        for (int e = 0; e < EY; ++e) {
+            v[e] = pf[e];
+            if (cg == 0) v[e] = cadd(v[e], mul_si<1>(colM[ty + TY * e]));
+        }
+        if (p + 1 < nplanes) prefetch(src_of(p + 1));
+        group_fft_ex<N, TY, 1, kWaveCols>(v, plane + (cact ? cg : 0), G::S, ty, stwN, cact);
+        __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITERS; ++it) {
             const int row = it * RPI + gx;
             const bool active = row < N;
             float2* rowp = plane + (active ? row : 0) * G::S;
-            float2 v[EX];
+            float2 vr[EX];
             if (active) {
                 const float2 c0 = rowp[0];   // (X0, XM), both real
 #pragma unroll
@@ -463,17 +492,18 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
                     const float2 xm = (k == 0) ? c2(c0.y, 0.f) : rowp[M - k];
                     const float2 xmc = cconj(xm);
                     const float2 d = cmul(csub(xk, xmc), cconj(stwN[k]));
-                    v[e] = cadd(cadd(xk, xmc), mul_si<1>(d));
+                    vr[e] = cadd(cadd(xk, xmc), mul_si<1>(d));
                 }
             }
-            __syncthreads();
-            group_fft<M, TX, 1, kWaveRows>(v, rowp, 1, tx, stwM, active);
+            // the row's own threads read it above and overwrite it below
+            group_sync<WROWS>();
+            group_fft<M, TX, 1, WROWS>(vr, rowp, 1, tx, stwM, active);
             if (active) {
 #pragma unroll
                 for (int e = 0; e < EX; ++e) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
-                        const float val = h ? v[e].y : v[e].x;
+                        const float val = h ? vr[e].y : vr[e].x;
                         if (g == 2) {
                             isd[it][e][h] = val;
                         } else if (g == 1) {
@@ -481,7 +511,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
                             const double a = (double)val * invV;
                             const double a2 = (double)isd[it][e][h] * invV * norm;
                             isd[it][e][h] = rsqrtf((float)(a2 - a * a));
-                        } else if ((mbits >> (2 * (it * EX + e) + h)) & 1u) {
+                        } else if ((mbits >> (2 * (it * EX + e) + h)) & MaskT(1)) {
                             // only S > 0 can beat the initial (0, 0); NaN never wins
                             const float sv = val * invVf * isd[it][e][h];
                             if (sv > 0.f) {
@@ -496,6 +526,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
         }
     }
 }
+
 
 // Brick occupancy for gather_pw: bit b set when any voxel of 8^3 brick b, or
 // of the one-voxel layer after it in +x/+y/+z (periodic), has a nonzero
@@ -525,6 +556,7 @@ __global__ void k_pack_tm(const float* __restrict__ t, const float* __restrict__
     if (i < n) tm[i] = c2(t[i], m[i]);
 }
 
+
 // ---------------------------------------------------------------------------
 // Host side: per-N configuration and dispatch.
 
@@ -549,18 +581,26 @@ int last(const char* what) {
     return kOk;
 }
 
-template <int N, int NTHR1, int TQ, int EMAXZ, int NTHR3>
+template <int N, int NTHR1, int TQ, int EMAXZ, int NTHR3, int EMAX = 16>
 int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     using G = Geo<N>;
     Tw tw;
     PF_CHECK_STATUS(get_tw(N, &tw));
+    // planes |z| <= zr only when the z stage treats the others as zero
+    const int zr = std::max(0, std::min(a.zr, N / 2));
+    const bool skip = a.skip_planes && 2 * zr + 1 < N;
+    const int nplanes = skip ? 2 * zr + 1 : N;
+    const int zlo = skip ? -zr : 0;
+    const int zr_read = skip ? zr : N;
     if (which < 0 || which == 0) {
         if (a.binary_mask)
-            hipLaunchKernelGGL((k_fwd_plane<N, NTHR1, true>), dim3(N, a.nrot), dim3(NTHR1), 0, s,
-                               a.tm, a.rots, a.rmax, a.work, tw.M, tw.N, a.occ);
+            hipLaunchKernelGGL((k_fwd_plane<N, NTHR1, true, EMAX>), dim3(nplanes, a.nrot),
+                               dim3(NTHR1), 0, s, a.tm, a.rots, a.rmax, a.zr, zlo, a.work, tw.M,
+                               tw.N, a.occ);
         else
-            hipLaunchKernelGGL((k_fwd_plane<N, NTHR1, false>), dim3(N, a.nrot), dim3(NTHR1), 0, s,
-                               a.tm, a.rots, a.rmax, a.work, tw.M, tw.N, a.occ);
+            hipLaunchKernelGGL((k_fwd_plane<N, NTHR1, false, EMAX>), dim3(nplanes, a.nrot),
+                               dim3(NTHR1), 0, s, a.tm, a.rots, a.rmax, a.zr, zlo, a.work, tw.M,
+                               tw.N, a.occ);
         PF_CHECK_STATUS(last("k_fwd_plane"));
     }
     if (which < 0 || which == 1) {
@@ -569,15 +609,15 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
         const int nthr = TQ * pick_T(N, EMAXZ, 32);
         if (a.binary_mask)
             hipLaunchKernelGGL((k_zcorr<N, TQ, EMAXZ, true>), grid, dim3(nthr), 0, s, a.work, a.F1,
-                               a.F2, a.nrot, ch, tw.N);
+                               a.F2, a.nrot, ch, zr_read, tw.N);
         else
             hipLaunchKernelGGL((k_zcorr<N, TQ, EMAXZ, false>), grid, dim3(nthr), 0, s, a.work,
-                               a.F1, a.F2, a.nrot, ch, tw.N);
+                               a.F1, a.F2, a.nrot, ch, zr_read, tw.N);
         PF_CHECK_STATUS(last("k_zcorr"));
     }
     if (which < 0 || which == 2) {
         const int ch = a.chunk_lcc;
-        hipLaunchKernelGGL((k_inv_plane<N, NTHR3>), dim3(N, (a.nrot + ch - 1) / ch),
+        hipLaunchKernelGGL((k_inv_plane<N, NTHR3, EMAX>), dim3(N, (a.nrot + ch - 1) / ch),
                            dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
                            a.norm, a.invV, tw.M, tw.N);
         PF_CHECK_STATUS(last("k_inv_plane"));
@@ -590,7 +630,7 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
 bool fused_supported(int nz, int ny, int nx) {
     if (nz != ny || ny != nx) return false;
     switch (nx) {
-        case 64: case 96: case 128: return true;
+        case 48: case 64: case 72: case 80: case 96: case 128: return true;
         default: return false;
     }
 }
@@ -606,7 +646,10 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         return kErrArg;
     }
     switch (a.n) {
+        case 48: return run_fused<48, 192, 32, 12, 192>(a, which, s);
         case 64: return run_fused<64, 256, 32, 8, 256>(a, which, s);
+        case 72: return run_fused<72, 384, 32, 12, 384>(a, which, s);
+        case 80: return run_fused<80, 320, 32, 16, 320, 12>(a, which, s);
         case 96: return run_fused<96, 384, 32, 12, 384>(a, which, s);
         case 128: return run_fused<128, 1024, 32, 8, 1024>(a, which, s);
         default:

@@ -84,6 +84,26 @@ constexpr int pick_T_min(int N, int emax, int tmin, int tmax) {
     return best_t ? best_t : pick_T(N, emax, tmax);
 }
 
+// Threads per column for M columns of N points done by NTHR threads: the T
+// (E = N / T <= emax) minimising passes x stages x (E + 4) — per-thread work
+// per stage plus a fixed exchange cost — with passes = ceil(M / (NTHR / T)).
+constexpr int pick_T_cols(int N, int M, int NTHR, int emax) {
+    int best_t = 0, best_cost = 1 << 30;
+    for (int T = 1; T <= 64 && T <= NTHR; ++T) {
+        if (N % T != 0 || N / T > emax || NTHR % T != 0) continue;
+        GPlan p = gplan_for(N, T);
+        if (p.ns < 0) continue;
+        const int cpi = NTHR / T;
+        const int passes = (M + cpi - 1) / cpi;
+        const int cost = passes * p.ns * (p.E + 4);
+        if (cost < best_cost || (cost == best_cost && T > best_t)) {
+            best_cost = cost;
+            best_t = T;
+        }
+    }
+    return best_t;
+}
+
 // 16-point DFT as 4 x 4 with inter-stage twiddles W16^(n2 k1).
 template <int DIR>
 PF_HD void dft16(float2* v) {
@@ -158,7 +178,7 @@ __device__ __forceinline__ void group_sync() {
 }
 #endif
 
-template <int N, int T, int DIR, int S, bool WAVE = false>
+template <int N, int T, int DIR, int S, bool WAVE = false, bool TO_EX = false>
 struct GroupStage {
     static constexpr GPlan P = gplan_for(N, T);
     static constexpr int E = N / T;
@@ -210,10 +230,13 @@ struct GroupStage {
                                                const float2* __restrict__ tw, bool active) {
         if constexpr (!kLast) {
             if (active) write_phase(v, ex, es, t, tw);
+            // the last stage's butterflies write the outputs at their natural
+            // positions: with TO_EX they stay there (no read-back, no sync)
+            if constexpr (TO_EX && S + 1 >= P.ns) return;
             group_sync<WAVE>();
             if (active) read_phase(v, ex, es, t);
             group_sync<WAVE>();
-            GroupStage<N, T, DIR, S + 1, WAVE>::run(v, ex, es, t, tw, active);
+            GroupStage<N, T, DIR, S + 1, WAVE, TO_EX>::run(v, ex, es, t, tw, active);
         }
     }
 #endif
@@ -223,7 +246,7 @@ struct GroupStage {
         if constexpr (!kLast) {
             for (int t = 0; t < T; ++t) write_phase(v[t], ex, 1, t, tw);
             for (int t = 0; t < T; ++t) read_phase(v[t], ex, 1, t);
-            GroupStage<N, T, DIR, S + 1, WAVE>::emulate(v, ex, tw);
+            GroupStage<N, T, DIR, S + 1, WAVE, TO_EX>::emulate(v, ex, tw);
         }
     }
 };
@@ -239,6 +262,18 @@ __device__ __forceinline__ void group_fft(float2 (&v)[N / T], float2* ex, int es
     static_assert(gplan_for(N, T).ns >= 0, "no radix plan for this N and T");
     static_assert(!WAVE || (64 % T == 0), "wave-local groups need T | 64");
     GroupStage<N, T, DIR, 0, WAVE>::run(v, ex, es, t, tw, active);
+}
+
+// As group_fft, but the transform is left in ex in natural order (element j
+// at ex[j * es]) and v holds no defined result.  The last stage ends without
+// a synchronisation: readers of ex from other threads must synchronise first
+// (a workgroup barrier, or group_sync<true> when they are in the same wave).
+template <int N, int T, int DIR, bool WAVE = false>
+__device__ __forceinline__ void group_fft_ex(float2 (&v)[N / T], float2* ex, int es, int t,
+                                             const float2* __restrict__ tw, bool active) {
+    static_assert(gplan_for(N, T).ns >= 1, "no radix plan for this N and T");
+    static_assert(!WAVE || (64 % T == 0), "wave-local groups need T | 64");
+    GroupStage<N, T, DIR, 0, WAVE, true>::run(v, ex, es, t, tw, active);
 }
 #endif
 

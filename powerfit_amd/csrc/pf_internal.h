@@ -97,6 +97,10 @@ struct FusedArgs {
     int nrot;                    // rotations in this batch
     int64_t first;               // global index of the first rotation
     int rmax;
+    int zr;                      // support half-width: rotated template and mask are zero
+                                 // where |signed z| > zr (and rows where z^2+y^2 > zr^2)
+    bool skip_planes;            // forward stage skips zero planes and the z stage reads
+                                 // them as zero (both stages fused)
     bool binary_mask;            // mask values all in {0, 1}: W^2 = W
     float2* work;                // [nrot][3][H] spectra
     const float2* F1;

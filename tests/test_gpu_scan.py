@@ -288,3 +288,28 @@ def test_serving_admission_respects_budget():
     assert len(res) == 3 and max(peak) == 1     # budget admits one fit at a time
     for r in res.values():
         assert np.array_equal(r.lcc, res[0].lcc)
+
+
+NEW_SIZES = {
+    48: Workload("f48", 48, 2.0, 600, 8.0, 20.0, seed=7500),
+    72: Workload("f72", 72, 1.8, 2000, 7.0, 20.0, laplace=True, core_weighted=True, seed=7600),
+    80: Workload("f80", 80, 1.5, 2500, 6.0, 20.0, seed=7700),
+}
+
+
+@pytest.mark.parametrize("n", sorted(NEW_SIZES))
+def test_fused_serving_sizes_vs_oracle(n):
+    w = NEW_SIZES[n]
+    c = make_case(w)
+    R = rotations_for_angle(w.angle)[:48]
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
+                           laplace=c.laplace, nproc=NPROC)
+    say(f"scan {n}^3 fused, 48 rotations")
+    corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused", batch=16)
+    assert corr.fused_available
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    corr.scan()
+    err, dec = assert_scan_parity(corr.lcc, corr.rot, ref)
+    say(f"  max err {err:.2e}, decisive voxels {dec}")

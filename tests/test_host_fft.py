@@ -38,3 +38,24 @@ def test_plans(host):
         ns = host.pf_host_plan(n, buf, 16)
         assert list(buf[:ns]) == expect
         assert int(np.prod(expect)) == n
+
+
+# (N, T) of every register group FFT the fused kernels instantiate: rows (N/2, TX),
+# columns and z (N, TY / TZ) for N in 48, 64, 72, 80, 96, 128, plus a few others
+GROUP_PAIRS = [(24, 2), (32, 4), (36, 3), (40, 4), (48, 4), (64, 8), (72, 6), (80, 8),
+               (96, 8), (128, 16), (128, 8), (64, 4), (96, 4), (160, 8), (192, 8), (100, 5)]
+
+
+@pytest.mark.parametrize("n,t", GROUP_PAIRS)
+def test_group_fft_matches_numpy(host, n, t):
+    """Register result (group_fft) and the exchange buffer after the last stage
+    (what group_fft_ex leaves in LDS) both equal numpy's transform."""
+    rng = np.random.default_rng(n * 100 + t)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    for d in (-1, 1):
+        ref = np.fft.fft(x.astype(np.complex128)) if d < 0 else np.fft.ifft(x) * n
+        for fn in (host.pf_host_group_fft, host.pf_host_group_fft_ex):
+            out = np.zeros(n, np.complex64)
+            ns = fn(n, t, d, x.ctypes.data_as(FP), out.ctypes.data_as(FP))
+            assert ns >= 1, (n, t)
+            assert np.abs(out - ref).max() / np.abs(ref).max() < 2e-6, (fn, n, t, d)

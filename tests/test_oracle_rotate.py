@@ -8,7 +8,7 @@ import pytest
 from scipy.ndimage import map_coordinates
 
 from oracle.rotate import rotate_grid3d, rotate_grid3d_np, sphere_points
-from powerfit_amd.rotations import load_set, quat_to_rotmat
+from powerfit_amd.rotations import load_set, quat_to_rotmat, rotations_for_angle
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -170,3 +170,27 @@ def test_against_reference_build_random():
                     assert np.array_equal(a, b)
                 else:
                     assert np.allclose(a, b, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("nearest", [False, True])
+def test_rotated_support_within_radius_plus_two(nearest):
+    """Rotation about voxel 0 preserves |v|: every voxel of the rotated grid
+    with |v| > ceil(R) + 2 is zero when the source is zero beyond radius R
+    (periodic signed coordinates).  The fused forward stage skips those planes
+    and rows, so this bound must hold for trilinear and nearest sampling."""
+    rng = np.random.default_rng(77)
+    shape = (24, 24, 24)
+    idx = [np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n) for n in shape]
+    z, y, x = np.meshgrid(*idx, indexing="ij")
+    d2 = z * z + y * y + x * x
+    allrot = rotations_for_angle(10.0)
+    rots = allrot[rng.choice(allrot.shape[0], 24, replace=False)]
+    for R in (3.3, 5.0, 7.9):
+        grid = np.where(d2 <= R * R, rng.random(shape) + 0.1, 0.0)
+        zr = int(np.ceil(np.sqrt(d2[grid != 0].max()))) + 2
+        for rot in rots:
+            out = np.zeros(shape)
+            rotate_grid3d(grid, rot, 12, out, nearest)
+            nz = out != 0
+            assert nz.any()
+            assert d2[nz].max() <= zr * zr
