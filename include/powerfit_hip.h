@@ -152,6 +152,34 @@ int pf_ctx_set_tuning(pf_ctx *ctx, int chunk_z, int chunk_lcc);
 int pf_ctx_profile(pf_ctx *ctx, int enable);
 int pf_ctx_kernel_time(pf_ctx *ctx, int which, double *ms, int64_t *launches);
 
+/* ---- template and mask generation (the step before the search) ----------
+ * Replace _powerfit.blur_points (_powerfit.pyx:75-140) and
+ * _powerfit.dilate_points (_powerfit.pyx:143-206), called by
+ * structure_to_shape_like (volume.py:192-224) for the CLI's template and mask
+ * (powerfit.py:245-255).  Same arguments: points is (3, n) row-major in grid
+ * units (x, y, z rows), out a (nz, ny, nx) grid updated in place (host
+ * memory); blur adds weights[k] exp(-d^2 / (2 sigma^2)) within 4 sigma of
+ * each atom, dilate sets 1 within radii[k]; an atom's box is clipped below at
+ * 0 (or at -dim + 1 with wraparound, negative indices wrapping once) and above
+ * at dim - 1.  Runs on `device` and returns when out holds the result. */
+int pf_blur_points(int device, const double *points, const double *weights, int64_t n,
+                   double sigma, double *out, int nz, int ny, int nx, int wraparound);
+int pf_dilate_points(int device, const double *points, const double *radii, int64_t n,
+                     double *out, int nz, int ny, int nx, int wraparound);
+
+/* ---- peak positions (the step after the search) ---------------------------
+ * Replaces the labelling in Analyzer._watershed (analyzer.py:80-96):
+ * scipy.ndimage.label(corr >= cutoff) with the 6-connected cross, no wrap,
+ * then scipy.ndimage.maximum_position over every label, for each of the
+ * ncut cutoffs.  corr and cutoffs are host arrays of dtype 0 = float32 or
+ * 1 = float64 (corr C-order (nz, ny, nx)).  positions receives the flat
+ * index of every component's maximum (the lowest index among equal maxima),
+ * cutoff after cutoff, up to capacity entries; *count is the number found,
+ * which may exceed capacity (call again with a larger buffer). */
+int pf_peak_positions(int device, const void *corr, int dtype, int nz, int ny, int nx,
+                      const void *cutoffs, int ncut, int64_t *positions, int64_t capacity,
+                      int64_t *count);
+
 #ifdef __cplusplus
 }
 #endif

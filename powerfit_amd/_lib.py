@@ -54,6 +54,11 @@ _SIGNATURES = {
     "pf_ctx_query": ([_vp], _c_int),
     "pf_ctx_bytes": ([_c_int, _c_int, _c_int, _c_int, _c_int], _c_i64),
     "pf_ctx_kernel_time": ([_vp, _c_int, _dp, ctypes.POINTER(_c_i64)], _c_int),
+    "pf_blur_points": ([_c_int, _dp, _dp, _c_i64, _c_dbl, _dp, _c_int, _c_int, _c_int, _c_int],
+                       _c_int),
+    "pf_dilate_points": ([_c_int, _dp, _dp, _c_i64, _dp, _c_int, _c_int, _c_int, _c_int], _c_int),
+    "pf_peak_positions": ([_c_int, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_int,
+                           ctypes.POINTER(_c_i64), _c_i64, ctypes.POINTER(_c_i64)], _c_int),
 }
 
 _lib = None

@@ -16,19 +16,7 @@ from os.path import abspath, isdir
 import numpy as np
 
 from .correlator import HipCorrelator
-
-
-class Volume:
-    """Minimal map container: density array, voxel spacing, origin."""
-
-    def __init__(self, array, voxelspacing=1.0, origin=(0, 0, 0)):
-        self.array = array
-        self.voxelspacing = voxelspacing
-        self.origin = origin
-
-    @property
-    def shape(self):
-        return self.array.shape
+from .volume import Volume  # noqa: F401  (re-exported: PowerFitter takes Volumes)
 
 
 def _array(v):
