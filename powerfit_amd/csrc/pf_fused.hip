@@ -40,8 +40,10 @@ constexpr int padded_stride(int M) {
 }
 
 // Row FFTs (groups of consecutive lanes, one wavefront) synchronise their
-// stages per wavefront; column FFTs run lanes across columns (conflict-free
-// LDS) with workgroup barriers.
+// stages per wavefront.  Column FFTs run lanes across columns (conflict-free
+// LDS, coalesced loads) with workgroup barriers, or, with kWaveCols, each
+// column's threads as consecutive lanes of one wavefront with wave-local
+// stage synchronisation.
 constexpr bool kWaveRows = true;
 constexpr bool kWaveCols = false;
 
@@ -218,6 +220,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     constexpr int EY = N / TY;
     constexpr int CPI = NTHR / TY;
     constexpr bool WROWS = kWaveRows && (64 % TX == 0);
+    constexpr bool WCOLS = kWaveCols && (64 % TY == 0);
     static_assert(TX > 0 && NTHR % TX == 0, "row plan");
     static_assert(TY > 0 && NTHR % TY == 0 && CPI >= M, "one column pass");
     __shared__ float2 plane[N * G::S];
@@ -233,7 +236,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     const int b = blockIdx.y;
     const int gx = threadIdx.x / TX, tx = threadIdx.x % TX;
     // column pass: lanes run across columns (conflict-free LDS)
-    const int cg = threadIdx.x % CPI, ty = threadIdx.x / CPI;
+    const int cg = WCOLS ? threadIdx.x / TY : threadIdx.x % CPI;
+    const int ty = WCOLS ? threadIdx.x % TY : threadIdx.x / CPI;
     const bool cact = cg < M;
     const int sz = signed_coord<N>(iz, r);
     double R[9];
@@ -309,7 +313,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                 }
             }
             __syncthreads();   // mirror reads are done before the FFT overwrites columns
-            group_fft_ex<N, TY, -1, kWaveCols>(v, plane + (cact ? cg : 0), G::S, ty, stwN, cact);
+            group_fft_ex<N, TY, -1, WCOLS>(v, plane + (cact ? cg : 0), G::S, ty, stwN, cact);
         }
         __syncthreads();
         store_plane<N, NTHR>(plane, dst + (int64_t)g * G::H);
@@ -407,6 +411,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
     constexpr int EY = N / TY;
     constexpr int CPI = NTHR / TY;
     constexpr bool WROWS = kWaveRows && (64 % TX == 0);
+    constexpr bool WCOLS = kWaveCols && (64 % TY == 0);
     static_assert(TY > 0 && NTHR % TY == 0 && CPI >= M, "one column pass");
     static_assert(NTHR >= N, "one Nyquist element per thread");
     static_assert(ITERS * EX * 2 <= 64, "owned voxels per thread must fit a 64-bit mask");
@@ -421,7 +426,8 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
     const int b1 = min(nrot, b0 + chunk);
     const int gx = threadIdx.x / TX, tx = threadIdx.x % TX;
     // column pass: lanes run across columns (coalesced loads, conflict-free LDS)
-    const int cg = threadIdx.x % CPI, ty = threadIdx.x / CPI;
+    const int cg = WCOLS ? threadIdx.x / TY : threadIdx.x % CPI;
+    const int ty = WCOLS ? threadIdx.x % TY : threadIdx.x / CPI;
     const bool cact = cg < M;
     const float invVf = (float)invV;
 
@@ -475,7 +481,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
             if (cg == 0) v[e] = cadd(v[e], mul_si<1>(colM[ty + TY * e]));
         }
         if (p + 1 < nplanes) prefetch(src_of(p + 1));
-        group_fft_ex<N, TY, 1, kWaveCols>(v, plane + (cact ? cg : 0), G::S, ty, stwN, cact);
+        group_fft_ex<N, TY, 1, WCOLS>(v, plane + (cact ? cg : 0), G::S, ty, stwN, cact);
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITERS; ++it) {
