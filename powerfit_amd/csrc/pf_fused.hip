@@ -534,6 +534,131 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
 }
 
 
+// ---------------------------------------------------------------------------
+// Two planes per pass (the forward planes of P and W for 0/1 masks).  The
+// column pass covers the 2M packed columns of both planes at once, so every
+// thread runs a longer register FFT with fewer LDS exchange stages, and the
+// pass needs half the workgroup barriers per plane; each thread transforms its
+// row of both planes.  Used where one column pass and one row pass cover both
+// planes and both fit in LDS.
+template <int N, int NTHR, int EMAX>
+struct PairPlan {
+    static constexpr int M = N / 2;
+    static constexpr int S = padded_stride(M + 1);
+    static constexpr int TY = pick_T_cols(N, 2 * M, NTHR, EMAX);
+    static constexpr int EY = TY > 0 ? N / TY : 1;
+    static constexpr int CPI = TY > 0 ? NTHR / TY : 0;
+    static constexpr int TX = pick_T(M, EMAX, 16);
+    static constexpr int EX = TX > 0 ? M / TX : 1;
+    static constexpr int RPI = TX > 0 ? NTHR / TX : 0;
+    static constexpr int NB = N / 8;
+    static constexpr int LDS = 8 * (2 * N * S + M + N) + 4 * ((NB * NB * NB + 31) / 32);
+    static constexpr bool ok = TY > 0 && NTHR % TY == 0 && CPI >= 2 * M && TX > 0 &&
+                               NTHR % TX == 0 && RPI >= N && NTHR >= N && EX * 2 <= 32 &&
+                               LDS <= 160 * 1024 - 512;
+};
+
+// Forward planes of P and W for 0/1 masks (W^2 = W): one gather fills the
+// rows of both, then one column pass covers both.
+template <int N, int NTHR, int EMAX>
+__global__ __launch_bounds__(NTHR) void k_fwd_plane2(const float2* __restrict__ tm,
+                                                     const double* __restrict__ rots, int r,
+                                                     int zr, int zlo, float2* __restrict__ out,
+                                                     const float2* __restrict__ twM,
+                                                     const float2* __restrict__ twN,
+                                                     const uint32_t* __restrict__ occ_g) {
+    using G = Geo<N>;
+    using P = PairPlan<N, NTHR, EMAX>;
+    constexpr int M = G::M, S = G::S, TY = P::TY, EY = P::EY, CPI = P::CPI, TX = P::TX,
+                  EX = P::EX;
+    constexpr int OCCW = (G::NB * G::NB * G::NB + 31) / 32;
+    constexpr bool WROWS = kWaveRows && (64 % TX == 0);
+    static_assert(P::ok, "pair plan");
+    __shared__ float2 planes[2 * N * S];
+    __shared__ uint32_t occ[OCCW];
+    __shared__ float2 stwM[M], stwN[N];
+    for (int i = threadIdx.x; i < OCCW; i += NTHR) occ[i] = occ_g[i];
+    stage_twiddles<M, NTHR>(stwM, twM);
+    stage_twiddles<N, NTHR>(stwN, twN);
+    __syncthreads();
+
+    const int iz = (blockIdx.x + zlo + N) % N;
+    const int b = blockIdx.y;
+    const int sz = signed_coord<N>(iz, r);
+    double R[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
+    float2* dst = out + ((int64_t)b * 3) * G::H + (int64_t)iz * G::PLANE;
+
+    {
+        // rows: gather the row of P and W, half-length x FFT of each into its plane
+        const int row = threadIdx.x / TX, tx = threadIdx.x % TX;
+        const int sy = signed_coord<N>(row, r);
+        const bool in_plane = row < N;
+        // rows whose every voxel lies outside the support are exactly zero
+        const bool active = in_plane && sz * sz + sy * sy <= zr * zr;
+        if (in_plane && !active) {
+#pragma unroll
+            for (int e = 0; e < EX; ++e) {
+                planes[row * S + tx + TX * e] = c2(0.f, 0.f);
+                planes[N * S + row * S + tx + TX * e] = c2(0.f, 0.f);
+            }
+        }
+        float2 vp[EX], vw[EX];
+        if (active) {
+#pragma unroll
+            for (int e = 0; e < EX; ++e) {
+                float p2[2], w2[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int x = signed_coord<N>(2 * (tx + TX * e) + h, r);
+                    p2[h] = 0.f;
+                    w2[h] = 0.f;
+                    if (sz * sz + sy * sy + x * x <= r * r)
+                        gather_pw<N>(tm, occ, R, sz, sy, x, &p2[h], &w2[h]);
+                }
+                vp[e] = c2(p2[0], p2[1]);
+                vw[e] = c2(w2[0], w2[1]);
+            }
+        }
+        float2* rowP = active ? planes + row * S : planes;
+        float2* rowW = active ? planes + N * S + row * S : planes;
+        group_fft_ex<M, TX, -1, WROWS>(vp, rowP, 1, tx, stwM, active);
+        group_fft_ex<M, TX, -1, WROWS>(vw, rowW, 1, tx, stwM, active);
+    }
+    __syncthreads();
+    {
+        // columns of both planes: R2C post-processing on the way in
+        // (X[k] = E[k] + w^k O[k], X0 and XM packed in column 0), y FFT left
+        // in the plane
+        const int cg = threadIdx.x % CPI, ty = threadIdx.x / CPI;
+        const int cs = cg / M, cc = cg - cs * M;
+        const bool cact = cs < 2;
+        float2* pl = planes + (cact ? cs : 0) * (N * S);
+        float2 v[EY];
+        if (cact) {
+#pragma unroll
+            for (int e = 0; e < EY; ++e) {
+                const float2* rowp = pl + (ty + TY * e) * S;
+                const float2 zk = rowp[cc];
+                if (cc == 0) {
+                    v[e] = c2(zk.x + zk.y, zk.x - zk.y);
+                } else {
+                    const float2 zm = cconj(rowp[M - cc]);
+                    const float2 Ev = cscale(cadd(zk, zm), 0.5f);
+                    const float2 Od = mul_si<-1>(cscale(csub(zk, zm), 0.5f));
+                    v[e] = cadd(Ev, cmul(stwN[cc], Od));
+                }
+            }
+        }
+        __syncthreads();   // mirror reads are done before the FFT overwrites columns
+        group_fft_ex<N, TY, -1, false>(v, pl + (cact ? cc : 0), S, ty, stwN, cact);
+    }
+    __syncthreads();
+    store_plane<N, NTHR>(planes, dst);
+    store_plane<N, NTHR>(planes + N * S, dst + G::H);
+}
+
 // Brick occupancy for gather_pw: bit b set when any voxel of 8^3 brick b, or
 // of the one-voxel layer after it in +x/+y/+z (periodic), has a nonzero
 // template or mask value.
@@ -587,6 +712,15 @@ int last(const char* what) {
     return kOk;
 }
 
+template <int N, int NTHR, int EMAX>
+void launch_fwd_pair(dim3 grid, hipStream_t s, const float2* tm, const double* rots, int r, int zr,
+                     int zlo, float2* out, const float2* twM, const float2* twN,
+                     const uint32_t* occ) {
+    if constexpr (PairPlan<N, NTHR, EMAX>::ok)
+        hipLaunchKernelGGL((k_fwd_plane2<N, NTHR, EMAX>), grid, dim3(NTHR), 0, s, tm, rots, r, zr,
+                           zlo, out, twM, twN, occ);
+}
+
 template <int N, int NTHR1, int TQ, int EMAXZ, int NTHR3, int EMAX = 16>
 int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     using G = Geo<N>;
@@ -599,7 +733,10 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     const int zlo = skip ? -zr : 0;
     const int zr_read = skip ? zr : N;
     if (which < 0 || which == 0) {
-        if (a.binary_mask)
+        if (a.binary_mask && PairPlan<N, NTHR1, EMAX>::ok)
+            launch_fwd_pair<N, NTHR1, EMAX>(dim3(nplanes, a.nrot), s, a.tm, a.rots, a.rmax, a.zr,
+                                            zlo, a.work, tw.M, tw.N, a.occ);
+        else if (a.binary_mask)
             hipLaunchKernelGGL((k_fwd_plane<N, NTHR1, true, EMAX>), dim3(nplanes, a.nrot),
                                dim3(NTHR1), 0, s, a.tm, a.rots, a.rmax, a.zr, zlo, a.work, tw.M,
                                tw.N, a.occ);
