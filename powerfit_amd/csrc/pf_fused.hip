@@ -659,6 +659,203 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane2(const float2* __restrict__ 
     store_plane<N, NTHR>(planes + N * S, dst + G::H);
 }
 
+// Copy a plane ([ky][kx] in global memory, NXH complex per row) into an LDS
+// slot with row stride S by direct global->LDS loads: each wave moves 64
+// consecutive floats of the slot per instruction (lane l -> float l of the
+// chunk), nothing passes through VGPRs.  Padding floats of a row receive a
+// copy of the row's first float.  The loads complete under vmcnt.
+template <int N, int NTHR>
+__device__ __forceinline__ void dma_plane(float2* slot, const float2* __restrict__ src) {
+    using G = Geo<N>;
+    constexpr int RF = 2 * G::S;     // floats per LDS row
+    constexpr int GF = 2 * G::NXH;   // floats per global row
+    constexpr int CHUNKS = N * RF / 64;
+    static_assert((N * RF) % 64 == 0, "whole 64-float chunks");
+    const float* s = reinterpret_cast<const float*>(src);
+    float* d = reinterpret_cast<float*>(slot);
+    const int lane = threadIdx.x & 63;
+    for (int c = threadIdx.x >> 6; c < CHUNKS; c += NTHR / 64) {
+        const int f = c * 64 + lane;
+        const int row = f / RF, fc = f - row * RF;
+        __builtin_amdgcn_global_load_lds(s + row * GF + (fc < GF ? fc : 0), d + c * 64, 4, 0, 0);
+    }
+}
+
+// Wait for this wave's outstanding global loads (LDS copies included) and
+// stores, then for the whole workgroup.
+__device__ __forceinline__ void wait_copies_and_sync() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
+// Inverse planes with the LCC, two LDS slots: per rotation one column pass
+// over the pair (B, A) (16-point column FFTs), their row passes (giving
+// 1/sqrt(N B - A^2) per voxel), then G with the one-plane column layout,
+// transformed from slot 0 into slot 1.  Planes reach LDS by direct copies
+// issued while the other slot is being transformed: G during the A rows, the
+// next B during G's transform; only the next A's copy is waited for.
+template <int N, int NTHR, int EMAX>
+__global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ X,
+                                                     const uint8_t* __restrict__ lmask,
+                                                     unsigned long long* __restrict__ keys,
+                                                     int nrot, int chunk, int64_t first,
+                                                     double norm, double invV,
+                                                     const float2* __restrict__ twM,
+                                                     const float2* __restrict__ twN) {
+    using G = Geo<N>;
+    using P = PairPlan<N, NTHR, EMAX>;
+    constexpr int M = G::M, S = G::S, TX = P::TX, EX = P::EX;
+    constexpr int TYP = P::TY, EYP = P::EY, CPIP = P::CPI;         // pair layout
+    constexpr int TYS = pick_T_cols(N, M, NTHR, EMAX);             // one-plane layout
+    constexpr int EYS = N / TYS, CPIS = NTHR / TYS;
+    constexpr bool WROWS = kWaveRows && (64 % TX == 0);
+    static_assert(P::ok, "pair plan");
+    static_assert(TYS > 0 && NTHR % TYS == 0 && CPIS >= M, "one column pass");
+    __shared__ float2 planes[2 * N * S];
+    __shared__ float2 stwM[M], stwN[N];
+    stage_twiddles<M, NTHR>(stwM, twM);
+    stage_twiddles<N, NTHR>(stwN, twN);
+    float2* const slot0 = planes;
+    float2* const slot1 = planes + N * S;
+
+    const int iz = blockIdx.x;
+    const int b0 = blockIdx.y * chunk;
+    const int b1 = min(nrot, b0 + chunk);
+    const int row = threadIdx.x / TX, tx = threadIdx.x % TX;   // row pass: one row
+    const bool ract = row < N;
+    const int cgp = threadIdx.x % CPIP, typ = threadIdx.x / CPIP;
+    const int csp = cgp / M, ccp = cgp - csp * M;               // pair: slot, column
+    const bool cactp = csp < 2;
+    const int ccs = threadIdx.x % CPIS, tys = threadIdx.x / CPIS;
+    const bool cacts = ccs < M;
+    const float invVf = (float)invV;
+
+    // owned voxels: (row, x = 2 (tx + TX e) + h), bit 2 e + h
+    uint32_t mbits = 0;
+    if (ract) {
+#pragma unroll
+        for (int e = 0; e < EX; ++e)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int64_t vox = ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
+                if (lmask[vox]) mbits |= 1u << (2 * e + h);
+            }
+    }
+    float isd[EX][2];
+
+    auto plane_of = [&](int b, int g) {
+        return X + ((int64_t)b * 3 + g) * G::H + (int64_t)iz * G::PLANE;
+    };
+    // C2R pre-processing of row `rowp` (packed (X0, XM) in column 0), then the
+    // row's half-length inverse FFT; the values end up in vr
+    auto row_inverse = [&](float2* rowp, float2 (&vr)[EX]) {
+        if (ract) {
+            const float2 c0 = rowp[0];
+#pragma unroll
+            for (int e = 0; e < EX; ++e) {
+                const int k = tx + TX * e;
+                const float2 xk = (k == 0) ? c2(c0.x, 0.f) : rowp[k];
+                const float2 xm = (k == 0) ? c2(c0.y, 0.f) : rowp[M - k];
+                const float2 xmc = cconj(xm);
+                const float2 d = cmul(csub(xk, xmc), cconj(stwN[k]));
+                vr[e] = cadd(cadd(xk, xmc), mul_si<1>(d));
+            }
+        }
+        // the row's own threads read it above and overwrite it below
+        group_sync<WROWS>();
+        group_fft<M, TX, 1, WROWS>(vr, rowp, 1, tx, stwM, ract);
+    };
+
+    if (b0 < b1) {
+        dma_plane<N, NTHR>(slot0, plane_of(b0, 2));   // B
+        dma_plane<N, NTHR>(slot1, plane_of(b0, 1));   // A
+    }
+    for (int b = b0; b < b1; ++b) {
+        wait_copies_and_sync();   // B and A of rotation b are in the slots
+        {
+            // y inverse of the packed columns of both slots: X0 + i XM in column 0
+            float2 v[EYP];
+            if (cactp) {
+                const float2* col = planes + csp * (N * S) + ccp;
+#pragma unroll
+                for (int e = 0; e < EYP; ++e) {
+                    v[e] = col[(typ + TYP * e) * S];
+                    if (ccp == 0) v[e] = cadd(v[e], mul_si<1>(col[(typ + TYP * e) * S + M]));
+                }
+            }
+            __syncthreads();   // every column is read before the FFT overwrites it
+            group_fft_ex<N, TYP, 1, false>(v, planes + (cactp ? csp * (N * S) + ccp : 0), S, typ,
+                                           stwN, cactp);
+        }
+        __syncthreads();
+        {
+            float2 vr[EX];
+            row_inverse(slot0 + (ract ? row : 0) * S, vr);
+            if (ract) {
+#pragma unroll
+                for (int e = 0; e < EX; ++e) {
+                    isd[e][0] = vr[e].x;   // B
+                    isd[e][1] = vr[e].y;
+                }
+            }
+        }
+        __syncthreads();   // slot 0 is free
+        dma_plane<N, NTHR>(slot0, plane_of(b, 0));   // G
+        {
+            float2 vr[EX];
+            row_inverse(slot1 + (ract ? row : 0) * S, vr);
+            if (ract) {
+#pragma unroll
+                for (int e = 0; e < EX; ++e)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        // N<W^2 T^2> - <W T>^2 in FP64, then 1/sqrt in FP32
+                        const double av = (double)(h ? vr[e].y : vr[e].x) * invV;
+                        const double a2 = (double)isd[e][h] * invV * norm;
+                        isd[e][h] = rsqrtf((float)(a2 - av * av));
+                    }
+            }
+        }
+        wait_copies_and_sync();   // G is in slot 0, slot 1 is free
+        {
+            float2 v[EYS];
+            if (cacts) {
+                const float2* col = slot0 + ccs;
+#pragma unroll
+                for (int e = 0; e < EYS; ++e) {
+                    v[e] = col[(tys + TYS * e) * S];
+                    if (ccs == 0) v[e] = cadd(v[e], mul_si<1>(col[(tys + TYS * e) * S + M]));
+                }
+            }
+            __syncthreads();   // slot 0 is read: the next B can come in
+            if (b + 1 < b1) dma_plane<N, NTHR>(slot0, plane_of(b + 1, 2));
+            group_fft_ex<N, TYS, 1, false>(v, slot1 + (cacts ? ccs : 0), S, tys, stwN, cacts);
+        }
+        __syncthreads();
+        {
+            float2 vr[EX];
+            row_inverse(slot1 + (ract ? row : 0) * S, vr);
+            if (ract) {
+#pragma unroll
+                for (int e = 0; e < EX; ++e)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (!((mbits >> (2 * e + h)) & 1u)) continue;
+                        // only S > 0 can beat the initial (0, 0); NaN never wins
+                        const float sv = (h ? vr[e].y : vr[e].x) * invVf * isd[e][h];
+                        if (sv > 0.f) {
+                            const int64_t vox =
+                                ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
+                            atomicMax(keys + vox, lcc_key(sv, (uint32_t)(first + b)));
+                        }
+                    }
+            }
+        }
+        __syncthreads();   // slot 1 is free
+        if (b + 1 < b1) dma_plane<N, NTHR>(slot1, plane_of(b + 1, 1));
+    }
+}
+
 // Brick occupancy for gather_pw: bit b set when any voxel of 8^3 brick b, or
 // of the one-voxel layer after it in +x/+y/+z (periodic), has a nonzero
 // template or mask value.
@@ -751,8 +948,8 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
         const dim3 grid((G::PLANE + TQ - 1) / TQ, (a.nrot + ch - 1) / ch);
         const int nthr = TQ * pick_T(N, EMAXZ, 32);
         if (a.binary_mask)
-            hipLaunchKernelGGL((k_zcorr<N, TQ, EMAXZ, true>), grid, dim3(nthr), 0, s, a.work, a.F1,
-                               a.F2, a.nrot, ch, zr_read, tw.N);
+            hipLaunchKernelGGL((k_zcorr<N, TQ, EMAXZ, true>), grid, dim3(nthr), 0, s, a.work,
+                               a.F1, a.F2, a.nrot, ch, zr_read, tw.N);
         else
             hipLaunchKernelGGL((k_zcorr<N, TQ, EMAXZ, false>), grid, dim3(nthr), 0, s, a.work,
                                a.F1, a.F2, a.nrot, ch, zr_read, tw.N);
@@ -760,9 +957,15 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     }
     if (which < 0 || which == 2) {
         const int ch = a.chunk_lcc;
-        hipLaunchKernelGGL((k_inv_plane<N, NTHR3, EMAX>), dim3(N, (a.nrot + ch - 1) / ch),
-                           dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
-                           a.norm, a.invV, tw.M, tw.N);
+        // two-slot inverse only for the large planes: it costs registers
+        if constexpr (PairPlan<N, NTHR3, EMAX>::ok && NTHR3 >= 512)
+            hipLaunchKernelGGL((k_inv_plane2<N, NTHR3, EMAX>), dim3(N, (a.nrot + ch - 1) / ch),
+                               dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
+                               a.norm, a.invV, tw.M, tw.N);
+        else
+            hipLaunchKernelGGL((k_inv_plane<N, NTHR3, EMAX>), dim3(N, (a.nrot + ch - 1) / ch),
+                               dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
+                               a.norm, a.invV, tw.M, tw.N);
         PF_CHECK_STATUS(last("k_inv_plane"));
     }
     return kOk;
