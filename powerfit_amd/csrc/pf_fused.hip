@@ -329,6 +329,8 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
     using G = Geo<N>;
     constexpr int TZ = pick_T(N, EMAX, 32);
     constexpr int EZ = N / TZ;
+    // lanes run across columns (coalesced column loads); stages use
+    // workgroup barriers (wave-local columns cost more in scattered loads)
     __shared__ float2 ex[N * TQ];
     __shared__ float2 stw[N];
     stage_twiddles<N, TQ * TZ>(stw, twN);

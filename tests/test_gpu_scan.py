@@ -105,11 +105,15 @@ def test_split_ranges_accumulate():
     assert np.array_equal(best, whole[0]) and np.array_equal(rot, whole[1])
 
 
-@pytest.mark.parametrize("name,nrot", [("cpu64", 96), ("lc96", 24), ("search128", 12)])
-def test_configs_rotation_subset(name, nrot):
+@pytest.mark.parametrize("name,nrot,mode", [("cpu64", 96, "batched"), ("lc96", 24, "batched"),
+                                            ("search128", 12, "batched"),
+                                            ("batch128", 12, "auto"), ("prec192", 6, "auto")])
+def test_configs_rotation_subset(name, nrot, mode):
+    """Each benchmark configuration (64^3 plain, 96^3 Laplace + core weighting,
+    128^3 at 10 and 5 degrees, 192^3) on a prefix of its rotation set."""
     c, R, ref = case_and_oracle(name, nrot=nrot)
-    say(f"scan {name} shape={c.target.shape} first {nrot} rotations")
-    best, rot = run_hip(c, R, "batched", 8)
+    say(f"scan {name} shape={c.target.shape} first {nrot} rotations, mode {mode}")
+    best, rot = run_hip(c, R, mode, 8)
     err, dec = assert_scan_parity(best, rot, ref)
     say(f"  max err {err:.2e}, decisive voxels {dec}")
 
