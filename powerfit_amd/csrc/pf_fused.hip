@@ -999,7 +999,7 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         case 72: return run_fused<72, 384, 32, 12, 384>(a, which, s);
         case 80: return run_fused<80, 320, 32, 16, 320, 12>(a, which, s);
         case 96: return run_fused<96, 384, 32, 12, 384>(a, which, s);
-        case 128: return run_fused<128, 1024, 32, 8, 1024>(a, which, s);
+        case 128: return run_fused<128, 1024, 32, 16, 1024>(a, which, s);
         default:
             set_error("fused pipeline not built for n = %d", a.n);
             return kErrUnsupported;
