@@ -86,12 +86,66 @@ def run_scan(corr, K, B, steps, first0=0):
     return first
 
 
+# ---- the CPU leg: the only place bench.py runs the oracle (FP64 numpy
+# restatement of the reference's CPU path) -- timed as the CPU baseline, and
+# as the checker of GPU results on samples
 def cpu_baseline(case, R, nproc, per_worker):
     from oracle import correlator as oc
     n = nproc * per_worker
     done, secs = oc.timed_parallel_scan(case.target.astype(np.float64), case.template, case.mask,
                                         R[:n], laplace=case.laplace, nproc=nproc)
     return done, secs
+
+
+def cpu_baseline_check(case, Rs, lcc, rot, nproc=1):
+    """Oracle result on rotations Rs and the GPU (lcc, rot) compared with it."""
+    from oracle import correlator as oc
+    if nproc > 1:
+        ref = oc.parallel_scan(case.target.astype(np.float64), case.template, case.mask, Rs,
+                               laplace=case.laplace, nproc=nproc)
+    else:
+        ref = oc.scan(case.target.astype(np.float64), case.template, case.mask, Rs,
+                      laplace=case.laplace)
+    decisive = ref.margin() > 1e-4
+    return {"max_abs_lcc_err": float(np.abs(lcc - ref.best).max()),
+            "index_mismatches": int(((rot != ref.rot.astype(np.int64)) & decisive).sum()),
+            "decisive_voxels": int(decisive.sum()), "tolerance": 1e-4}
+
+
+def precision_report(args, HipCorrelator, case, R, w):
+    """FP32 vs split-precision (MFMA first stage) unfused passes on a rotation
+    prefix: rotations/s with inputs resident, and both results checked against
+    the FP64 oracle in the CPU leg."""
+    n = min(args.precision_rotations, R.shape[0])
+    Rs = R[:n]
+    rows = {}
+    results = {}
+    for prec in ("fp32", "split"):
+        corr = HipCorrelator(case.target.astype(np.float64), laplace=case.laplace, device=0,
+                             mode="batched", batch=args.precision_batch, precision=prec)
+        corr.template = case.template
+        corr.mask = case.mask
+        corr.rotations = Rs
+        corr.scan()                      # warm-up, also the result checked below
+        results[prec] = (corr.lcc.copy(), corr.rot.copy())
+        best = float("inf")
+        for _ in range(2):
+            corr.reset()
+            corr.sync()
+            t = time.perf_counter()
+            corr.enqueue_scan(0, n)
+            corr.sync()
+            best = min(best, time.perf_counter() - t)
+        rows[prec] = {"rotations_per_s": n / best}
+        del corr
+    nproc = min(16, os.cpu_count() or 1)
+    for prec in ("fp32", "split"):
+        rows[prec].update(cpu_baseline_check(case, Rs, *results[prec], nproc=nproc))
+    nz, ny, nx = case.target.shape
+    return {"workload": f"{w.name}: {nz}x{ny}x{nx}, first {n} rotations of the {w.angle} deg set",
+            "path": "unfused batched passes, batch %d" % args.precision_batch,
+            "fp32": rows["fp32"], "split": rows["split"],
+            "split_vs_fp32_rate": rows["split"]["rotations_per_s"] / rows["fp32"]["rotations_per_s"]}
 
 
 def main(argv=None):
@@ -112,6 +166,15 @@ def main(argv=None):
                          "counters under a profiler)")
     ap.add_argument("--cpu-per-worker", type=int, default=2,
                     help="rotations per CPU worker in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "split"],
+                    help="FFT arithmetic of the measured path: fp32, or split (first radix-16 "
+                         "stage of the unfused y/z passes as an FP16 hi+lo GEMM on MFMA)")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="also compare FP32 and split-precision (MFMA first stage) FFT passes "
+                         "on --precision-workload against the FP64 oracle")
+    ap.add_argument("--precision-workload", default="prec192")
+    ap.add_argument("--precision-rotations", type=int, default=32)
+    ap.add_argument("--precision-batch", type=int, default=8)
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -141,7 +204,7 @@ def main(argv=None):
     H = nz * ny * (nx // 2 + 1)
 
     corr = HipCorrelator(case.target.astype(np.float64), laplace=case.laplace, device=0,
-                         mode=args.mode, batch=max(1, args.batch))
+                         mode=args.mode, batch=max(1, args.batch), precision=args.precision)
     corr.template = case.template
     corr.mask = case.mask
     corr.rotations = R
@@ -202,10 +265,9 @@ def main(argv=None):
         plain_value = args.plain_rotations / (time.perf_counter() - t)
         del plain
 
-    # parity on a sample: the FP64 oracle vs the same path on the same rotations
+    # parity on a sample: the FP64 oracle (CPU leg) vs the same path on the same rotations
     parity = None
     if args.parity_rotations > 0:
-        from oracle import correlator as oc
         best_full = np.zeros(case.target.shape, np.float32)
         rot_full = np.zeros(case.target.shape, np.int32)
         corr.reset()
@@ -217,18 +279,14 @@ def main(argv=None):
         pick = sorted(set(range(min(args.parity_rotations - 1, K))) | {peak_rot})
         Rs = R[pick]
         sample = HipCorrelator(case.target.astype(np.float64), laplace=case.laplace, device=0,
-                               mode=args.mode, batch=max(1, min(B, len(pick))))
+                               mode=args.mode, batch=max(1, min(B, len(pick))),
+                               precision=args.precision)
         sample.template = case.template
         sample.mask = case.mask
         sample.rotations = Rs
         sample.scan()
-        ref = oc.scan(case.target.astype(np.float64), case.template, case.mask, Rs,
-                      laplace=case.laplace)
-        decisive = ref.margin() > 1e-4
-        parity = {"rotations": pick, "max_abs_lcc_err": float(np.abs(sample.lcc - ref.best).max()),
-                  "index_mismatches": int(((sample.rot != ref.rot.astype(np.int64))
-                                           & decisive).sum()),
-                  "decisive_voxels": int(decisive.sum()), "tolerance": 1e-4,
+        parity = {"rotations": pick,
+                  **cpu_baseline_check(case, Rs, sample.lcc, sample.rot.astype(np.int64)),
                   "full_scan_peak_lcc": float(best_full.max())}
         del sample
 
@@ -260,7 +318,8 @@ def main(argv=None):
                    "grid": [nz, ny, nx], "rotations": K, "batch": B, "mode": args.mode,
                    "autotuned": None if tuning is None else
                    {k: tuning[k] for k in ("batch", "chunk_z", "chunk_lcc", "trials")},
-                   "laplace": w.laplace, "core_weighted": w.core_weighted},
+                   "laplace": w.laplace, "core_weighted": w.core_weighted,
+                   "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "alg_bytes_per_launch": alg_bytes_per_launch(dom, V, H, B, binary),
@@ -275,6 +334,10 @@ def main(argv=None):
         "cpu_baseline": cpu,
         "parity": parity,
     }
+    if args.precision_report:
+        pw = WORKLOADS[args.precision_workload]
+        out["precision"] = precision_report(args, HipCorrelator, make_case(pw),
+                                            rotations_for_angle(pw.angle), pw)
     print(json.dumps(out))
     return 0
 

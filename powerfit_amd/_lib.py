@@ -17,6 +17,7 @@ MODES = {"plain": 0, "batched": 1, "fused": 2, "auto": 3,
 MODE_PLAIN = MODES["plain"]
 MODE_BATCHED = MODES["batched"]
 FLAG_NO_BINARY_MASK = 1
+FLAG_SPLIT_DFT = 2
 KERNELS = ["rotate", "r2c_x", "c2c_fwd", "conj_mul", "c2c_inv", "c2r_x", "lcc",
            "fwd_plane", "zcorr", "inv_plane"]
 

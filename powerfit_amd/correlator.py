@@ -114,13 +114,20 @@ class HipCorrelator(BaseCorrelator):
             fused stage with the rest unfused (for testing).
     batch : rotations per launch.
     binary_shortcut : use FFT(W^2) = FFT(W) when the mask is 0/1.
+    precision : "fp32" (default) or "split": the unfused y and z FFT passes
+        do their first radix-16 stage as a DFT-as-GEMM on the matrix cores with
+        FP16 hi + lo operands and FP32 accumulation (ny, nz divisible by 16 and
+        at most 320; the fused pipeline is not used).
     """
 
     def __init__(self, target, laplace=False, device=0, mode="auto", batch=64,
-                 binary_shortcut=True):
+                 binary_shortcut=True, precision="fp32"):
         super().__init__(target, laplace=laplace)
         if mode not in _lib.MODES:
             raise ValueError(f"unknown mode {mode!r}")
+        if precision not in ("fp32", "split"):
+            raise ValueError(f"unknown precision {precision!r}")
+        self.precision = precision
         self._dev = device
         self.mode = mode
         self.batch = int(batch)
@@ -129,6 +136,8 @@ class HipCorrelator(BaseCorrelator):
         ctx = ctypes.c_void_p()
         nz, ny, nx = self._target.shape
         flags = 0 if binary_shortcut else _lib.FLAG_NO_BINARY_MASK
+        if precision == "split":
+            flags |= _lib.FLAG_SPLIT_DFT
         _lib.check(lib.pf_ctx_create(device, nz, ny, nx, flags, ctypes.byref(ctx)),
                    "pf_ctx_create")
         self._ctx = ctx

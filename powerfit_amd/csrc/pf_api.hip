@@ -248,7 +248,7 @@ int pf_irfftn_dev(void* in, float* out, int nz, int ny, int nx, int batch, void*
 int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** out) {
     if (!out) return arg_error("out is null");
     *out = nullptr;
-    if (flags & ~PF_FLAG_NO_BINARY_MASK) return arg_error("unknown flags");
+    if (flags & ~(PF_FLAG_NO_BINARY_MASK | PF_FLAG_SPLIT_DFT)) return arg_error("unknown flags");
     if (nz < 1 || ny < 1 || nx < 1) return arg_error("invalid grid shape");
     int ndev = 0;
     PF_HIP_CHECK(hipGetDeviceCount(&ndev));
@@ -269,6 +269,7 @@ int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** o
         if (c->num_cu <= 0) c->num_cu = 256;
     }
     if (!s) s = fft3d_init(&c->fft, nz, ny, nx);
+    if (!s && (flags & PF_FLAG_SPLIT_DFT)) s = fft3d_set_split(&c->fft, true);
     if (!s && hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
         s = kErrHip;
@@ -448,10 +449,15 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     if (first < 0 || count < 0 || first + count > c->nrot)
         return arg_error("rotation range outside the rotation set");
     if (mode < PF_MODE_PLAIN || mode > PF_MODE_MIX_INV) return arg_error("unknown mode");
-    const bool fused_ok = fused_supported(c->nz, c->ny, c->nx) && c->rmax <= c->nx / 2;
+    // the split-precision FFT stage exists in the unfused passes only
+    const bool split = (c->flags & PF_FLAG_SPLIT_DFT) != 0;
+    const bool fused_ok =
+        !split && fused_supported(c->nz, c->ny, c->nx) && c->rmax <= c->nx / 2;
     if (mode == PF_MODE_AUTO) mode = fused_ok ? PF_MODE_FUSED : PF_MODE_BATCHED;
     if (mode >= PF_MODE_FUSED && !fused_ok) {
-        set_error("fused pipeline not available for grid (%d, %d, %d)", c->nz, c->ny, c->nx);
+        set_error(split ? "fused pipeline not available with PF_FLAG_SPLIT_DFT"
+                        : "fused pipeline not available for grid (%d, %d, %d)",
+                  c->nz, c->ny, c->nx);
         return kErrUnsupported;
     }
     if (mode == PF_MODE_PLAIN) batch = 1;
@@ -506,8 +512,8 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
                                     c->stream);
             }));
             PF_CHECK_STATUS(timed(c, PF_K_C2C_FWD, [&] {
-                return launch_c2c_axis(c->fft.py, c->fft.twy, c->d_cplx, c->fft.nxh, c->fft.nxh,
-                                       (int64_t)3 * nb * c->nz, plane, -1, c->stream);
+                return launch_axis(c->fft, 1, c->d_cplx, c->fft.nxh, c->fft.nxh,
+                                   (int64_t)3 * nb * c->nz, plane, -1, c->stream);
             }));
         }
         if (fb) {
@@ -515,23 +521,23 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
             PF_CHECK_STATUS(timed(c, PF_K_ZCORR, [&] { return launch_fused(fa_args, 1, c->stream); }));
         } else {
             PF_CHECK_STATUS(timed(c, PF_K_C2C_FWD, [&] {
-                return launch_c2c_axis(c->fft.pz, c->fft.twz, c->d_cplx, plane, plane,
-                                       (int64_t)3 * nb, plane * c->nz, -1, c->stream);
+                return launch_axis(c->fft, 2, c->d_cplx, plane, plane, (int64_t)3 * nb,
+                                   plane * c->nz, -1, c->stream);
             }));
             PF_CHECK_STATUS(timed(c, PF_K_CONJ_MUL, [&] {
                 return launch_conj_mul_pw(c->d_cplx, c->d_F1, c->d_F2, c->H, nb, c->stream);
             }));
             PF_CHECK_STATUS(timed(c, PF_K_C2C_INV, [&] {
-                return launch_c2c_axis(c->fft.pz, c->fft.twz, c->d_cplx, plane, plane,
-                                       (int64_t)3 * nb, plane * c->nz, 1, c->stream);
+                return launch_axis(c->fft, 2, c->d_cplx, plane, plane, (int64_t)3 * nb,
+                                   plane * c->nz, 1, c->stream);
             }));
         }
         if (fc) {
             PF_CHECK_STATUS(timed(c, PF_K_INV_PLANE, [&] { return launch_fused(fa_args, 2, c->stream); }));
         } else {
             PF_CHECK_STATUS(timed(c, PF_K_C2C_INV, [&] {
-                return launch_c2c_axis(c->fft.py, c->fft.twy, c->d_cplx, c->fft.nxh, c->fft.nxh,
-                                       (int64_t)3 * nb * c->nz, plane, 1, c->stream);
+                return launch_axis(c->fft, 1, c->d_cplx, c->fft.nxh, c->fft.nxh,
+                                   (int64_t)3 * nb * c->nz, plane, 1, c->stream);
             }));
             PF_CHECK_STATUS(timed(c, PF_K_C2R_X, [&] {
                 return launch_c2r_x(c->fft, c->d_cplx, c->d_real, (int64_t)3 * nb * c->nz * c->ny,
@@ -625,7 +631,10 @@ int pf_scan_graph(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) 
 
 int pf_ctx_fused_available(pf_ctx* c) {
     if (!c) return arg_error("ctx is null");
-    return (fused_supported(c->nz, c->ny, c->nx) && (c->rmax < 0 || c->rmax <= c->nx / 2)) ? 1 : 0;
+    return (!(c->flags & PF_FLAG_SPLIT_DFT) && fused_supported(c->nz, c->ny, c->nx) &&
+            (c->rmax < 0 || c->rmax <= c->nx / 2))
+               ? 1
+               : 0;
 }
 
 int pf_get_result(pf_ctx* c, float* best, int32_t* idx) {

@@ -47,8 +47,23 @@ struct Fft3D {
     int nz, ny, nx, nxh;
     Plan1D px, py, pz;         // px: length nx/2 (even nx) or nx (odd nx)
     const float2 *twx, *twy, *twz, *twfull;  // twfull: length nx (R2C post-processing)
+    // split-precision mode: the y and z passes do their first radix-16 stage
+    // as a DFT-as-GEMM on MFMA (plans py16, pz16 start with radix 16)
+    bool split = false;
+    Plan1D py16{}, pz16{};
 };
 int fft3d_init(Fft3D* f, int nz, int ny, int nx);
+// Turn the split-precision y/z passes on (both lengths divisible by 16, at
+// most kSplitMaxLen) or off.
+int fft3d_set_split(Fft3D* f, bool on);
+constexpr int kSplitMaxLen = 320;   // two 16-column LDS tiles of complex64: 80 KB at 320
+bool make_plan16(int n, Plan1D* p);
+int launch_c2c_axis_split(const Plan1D& p, const float2* tw, float2* data, int64_t stride,
+                          int64_t inner, int64_t outer, int64_t outer_stride, int dir,
+                          hipStream_t s);
+// The y (axis 1) or z (axis 2) C2C pass of f, split-precision when f.split.
+int launch_axis(const Fft3D& f, int axis, float2* data, int64_t stride, int64_t inner,
+                int64_t outer, int64_t outer_stride, int dir, hipStream_t s);
 int fft3d_r2c(const Fft3D& f, const float* in, float2* out, int batch, hipStream_t s);
 int fft3d_c2r(const Fft3D& f, float2* in, float* out, int batch, float scale, hipStream_t s);
 

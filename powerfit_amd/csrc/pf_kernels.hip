@@ -425,20 +425,42 @@ int launch_c2c_axis(const Plan1D& p, const float2* tw, float2* data, int64_t str
     return check_launch("k_c2c_axis");
 }
 
+int fft3d_set_split(Fft3D* f, bool on) {
+    if (!on) {
+        f->split = false;
+        return kOk;
+    }
+    if (!make_plan16(f->ny, &f->py16) || !make_plan16(f->nz, &f->pz16)) {
+        set_error("split-precision FFT needs ny and nz divisible by 16 and at most %d "
+                  "(got %d, %d)", kSplitMaxLen, f->ny, f->nz);
+        return kErrUnsupported;
+    }
+    f->split = true;
+    return kOk;
+}
+
+int launch_axis(const Fft3D& f, int axis, float2* data, int64_t stride, int64_t inner,
+                int64_t outer, int64_t outer_stride, int dir, hipStream_t s) {
+    const float2* tw = axis == 1 ? f.twy : f.twz;
+    if (f.split)
+        return launch_c2c_axis_split(axis == 1 ? f.py16 : f.pz16, tw, data, stride, inner, outer,
+                                     outer_stride, dir, s);
+    return launch_c2c_axis(axis == 1 ? f.py : f.pz, tw, data, stride, inner, outer,
+                           outer_stride, dir, s);
+}
+
 int fft3d_r2c(const Fft3D& f, const float* in, float2* out, int batch, hipStream_t s) {
     const int64_t plane = (int64_t)f.ny * f.nxh;
     PF_CHECK_STATUS(launch_r2c_x(f, in, out, (int64_t)batch * f.nz * f.ny, s));
-    PF_CHECK_STATUS(launch_c2c_axis(f.py, f.twy, out, f.nxh, f.nxh, (int64_t)batch * f.nz,
-                                    plane, -1, s));
-    PF_CHECK_STATUS(launch_c2c_axis(f.pz, f.twz, out, plane, plane, batch, plane * f.nz, -1, s));
+    PF_CHECK_STATUS(launch_axis(f, 1, out, f.nxh, f.nxh, (int64_t)batch * f.nz, plane, -1, s));
+    PF_CHECK_STATUS(launch_axis(f, 2, out, plane, plane, batch, plane * f.nz, -1, s));
     return kOk;
 }
 
 int fft3d_c2r(const Fft3D& f, float2* in, float* out, int batch, float scale, hipStream_t s) {
     const int64_t plane = (int64_t)f.ny * f.nxh;
-    PF_CHECK_STATUS(launch_c2c_axis(f.pz, f.twz, in, plane, plane, batch, plane * f.nz, 1, s));
-    PF_CHECK_STATUS(launch_c2c_axis(f.py, f.twy, in, f.nxh, f.nxh, (int64_t)batch * f.nz,
-                                    plane, 1, s));
+    PF_CHECK_STATUS(launch_axis(f, 2, in, plane, plane, batch, plane * f.nz, 1, s));
+    PF_CHECK_STATUS(launch_axis(f, 1, in, f.nxh, f.nxh, (int64_t)batch * f.nz, plane, 1, s));
     PF_CHECK_STATUS(launch_c2r_x(f, in, out, (int64_t)batch * f.nz * f.ny, scale, s));
     return kOk;
 }
