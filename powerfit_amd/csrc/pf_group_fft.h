@@ -33,10 +33,12 @@ constexpr GPlan gplan_for(int N, int T) {
     }
     p.E = N / T;
     int rem = N;
-    const int pref[7] = {16, 8, 4, 2, 3, 5, 7};
+    // largest first (fewer exchange stages); composite radices run in
+    // registers as dft_ab
+    const int pref[12] = {16, 24, 12, 20, 8, 10, 6, 4, 2, 3, 5, 7};
     while (rem > 1) {
         int pick = 0;
-        for (int i = 0; i < 7; ++i) {
+        for (int i = 0; i < 12; ++i) {
             const int R = pref[i];
             if (rem % R == 0 && p.E % R == 0) {
                 pick = R;
@@ -140,8 +142,52 @@ PF_HD void dft16(float2* v) {
 }
 
 template <int R, int DIR>
+PF_HD void dft_any(float2* v);
+
+// R = A B point DFT in registers (Cooley-Tukey): A-point DFTs over
+// x[B n1 + n2], twiddles W_R^(n2 k1), then B-point DFTs -> X[k1 + A k2].
+// Lets a thread holding E elements use a composite radix dividing E, so a
+// transform needs fewer LDS exchange stages.
+template <int A, int B, int DIR>
+PF_HD void dft_ab(float2* v) {
+    constexpr int R = A * B;
+    float2 y[B][A];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+        float2 u[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) u[n1] = v[B * n1 + n2];
+        dft_any<A, DIR>(u);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) {
+            const int m = (n2 * k1) % R;
+            if (m == 0) {
+                y[n2][k1] = u[k1];
+            } else {
+                const double a = (double)DIR * 2.0 * 3.14159265358979323846 * m / R;
+                y[n2][k1] = cmul(u[k1], c2((float)__builtin_cos(a), (float)__builtin_sin(a)));
+            }
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+        float2 w[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) w[n2] = y[n2][k1];
+        dft_any<B, DIR>(w);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = w[k2];
+    }
+}
+
+template <int R, int DIR>
 PF_HD void dft_any(float2* v) {
     if constexpr (R == 16) dft16<DIR>(v);
+    else if constexpr (R == 6) dft_ab<2, 3, DIR>(v);
+    else if constexpr (R == 10) dft_ab<2, 5, DIR>(v);
+    else if constexpr (R == 12) dft_ab<4, 3, DIR>(v);
+    else if constexpr (R == 20) dft_ab<4, 5, DIR>(v);
+    else if constexpr (R == 24) dft_ab<8, 3, DIR>(v);
     else if constexpr (R == 7) {
         // 7-point DFT directly (rare radix)
         float2 x[7];
