@@ -317,7 +317,8 @@ def main(argv=None):
                                f"rotation set (K={K})",
                    "grid": [nz, ny, nx], "rotations": K, "batch": B, "mode": args.mode,
                    "autotuned": None if tuning is None else
-                   {k: tuning[k] for k in ("batch", "chunk_z", "chunk_lcc", "trials")},
+                   {k: tuning.get(k) for k in ("batch", "chunk_z", "chunk_lcc", "z_schedule",
+                                                "trials")},
                    "laplace": w.laplace, "core_weighted": w.core_weighted,
                    "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -55,6 +55,7 @@ _SIGNATURES = {
     "pf_ctx_query": ([_vp], _c_int),
     "pf_ctx_bytes": ([_c_int, _c_int, _c_int, _c_int, _c_int], _c_i64),
     "pf_ctx_kernel_time": ([_vp, _c_int, _dp, ctypes.POINTER(_c_i64)], _c_int),
+    "pf_ctx_set_schedule": ([_vp, _c_int], _c_int),
     "pf_blur_points": ([_c_int, _dp, _dp, _c_i64, _c_dbl, _dp, _c_int, _c_int, _c_int, _c_int],
                        _c_int),
     "pf_dilate_points": ([_c_int, _dp, _dp, _c_i64, _dp, _c_int, _c_int, _c_int, _c_int], _c_int),

@@ -1,11 +1,13 @@
-"""Launch-shape autotuner for the LCC search.
+"""Launch-shape and FFT-schedule autotuner for the LCC search.
 
 Times short scans of the correlator's own inputs over candidate rotation-batch
-sizes and per-block rotation chunks of the fused kernels, and keeps the
-fastest.  Results are cached per (device, grid, mask kind, mode) in a JSON
-file (``$POWERFIT_AMD_TUNE`` or ``~/.cache/powerfit_amd/tune.json``).  The
-search result does not depend on these choices: every rotation's LCC map is
-computed identically and merged with an order-independent max.
+sizes and per-block rotation chunks of the fused kernels, then, at the best
+of those, the alternative radix schedule of the fused z-stage FFTs, and keeps
+the fastest.  Results are cached per (device, grid, mask kind, mode) in a JSON
+file (``$POWERFIT_AMD_TUNE`` or ``~/.cache/powerfit_amd/tune.json``).  Batch
+and chunk sizes do not change the result (every rotation's LCC map is computed
+identically and merged with an order-independent max); the FFT schedule
+changes it only by FP32 rounding.
 """
 import json
 import os
@@ -37,9 +39,10 @@ def _key(corr):
            f"{'binary' if binary else 'weighted'}|{corr.mode}"
 
 
-def _time(corr, batch, chunks, nrot, repeats=2):
+def _time(corr, batch, chunks, nrot, repeats=2, z_schedule=0):
     corr.batch = batch
     corr.set_tuning(*chunks)
+    corr.set_schedule(z_schedule)
     K = corr.rotations.shape[0]
     n = min(nrot, K)
     corr.reset()
@@ -56,11 +59,12 @@ def _time(corr, batch, chunks, nrot, repeats=2):
 
 
 def autotune(corr, batches=DEFAULT_BATCHES, chunks=DEFAULT_CHUNKS, nrot=None, use_cache=True,
-             verbose=False):
-    """Pick (batch, chunk_z, chunk_lcc) for ``corr`` (a ready HipCorrelator).
+             verbose=False, schedules=(0, 1)):
+    """Pick (batch, chunk_z, chunk_lcc, z_schedule) for ``corr`` (a ready
+    HipCorrelator).
 
     Applies the choice to ``corr`` and returns
-    ``{"batch", "chunk_z", "chunk_lcc", "rate", "trials"}``.
+    ``{"batch", "chunk_z", "chunk_lcc", "z_schedule", "rate", "trials"}``.
     """
     path = _cache_path()
     key = _key(corr)
@@ -75,6 +79,7 @@ def autotune(corr, batches=DEFAULT_BATCHES, chunks=DEFAULT_CHUNKS, nrot=None, us
         hit = cache[key]
         corr.batch = hit["batch"]
         corr.set_tuning(hit["chunk_z"], hit["chunk_lcc"])
+        corr.set_schedule(hit.get("z_schedule", 0))
         return dict(hit, cached=True)
     nrot = nrot or 2 * max(batches)
     trials = []
@@ -88,8 +93,22 @@ def autotune(corr, batches=DEFAULT_BATCHES, chunks=DEFAULT_CHUNKS, nrot=None, us
                 print(f"autotune {key}: batch {b} chunks ({cz},{cl}) -> {rate:.0f} rot/s",
                       flush=True)
     best = max(trials, key=lambda t: t["rate"])
+    best["z_schedule"] = 0
+    if schedules and corr.fused_available and corr.mode in ("auto", "fused"):
+        for zs in schedules:
+            if zs == 0:
+                continue
+            rate = _time(corr, best["batch"], (best["chunk_z"], best["chunk_lcc"]), nrot,
+                         z_schedule=zs)
+            trials.append({"batch": best["batch"], "chunk_z": best["chunk_z"],
+                           "chunk_lcc": best["chunk_lcc"], "z_schedule": zs, "rate": rate})
+            if verbose:
+                print(f"autotune {key}: z schedule {zs} -> {rate:.0f} rot/s", flush=True)
+            if rate > best["rate"]:
+                best = dict(trials[-1])
     corr.batch = best["batch"]
     corr.set_tuning(best["chunk_z"], best["chunk_lcc"])
+    corr.set_schedule(best["z_schedule"])
     out = dict(best, trials=len(trials))
     if use_cache:
         cache[key] = out

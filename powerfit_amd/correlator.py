@@ -131,6 +131,7 @@ class HipCorrelator(BaseCorrelator):
         self._dev = device
         self.mode = mode
         self.batch = int(batch)
+        self.z_schedule = 0
         lib = _lib.load()
         self._libref = lib
         ctx = ctypes.c_void_p()
@@ -218,6 +219,12 @@ class HipCorrelator(BaseCorrelator):
         """Rotations per block in the fused z and LCC kernels (0 = automatic)."""
         _lib.check(self._libref.pf_ctx_set_tuning(self._ctx, int(chunk_z), int(chunk_lcc)),
                    "pf_ctx_set_tuning")
+
+    def set_schedule(self, z_schedule=0):
+        """Radix schedule of the fused z-stage FFTs (0 default, 1 alternative)."""
+        _lib.check(self._libref.pf_ctx_set_schedule(self._ctx, int(z_schedule)),
+                   "pf_ctx_set_schedule")
+        self.z_schedule = int(z_schedule)
 
     @property
     def fused_available(self):

@@ -30,12 +30,13 @@ struct pf_ctx {
     uint32_t flags = 0;
     bool mask_binary = false;
     int chunk_z = 0, chunk_lcc = 0;   // 0 = automatic
+    int zsched = 0;                   // z-FFT schedule of the fused z stage
     int support_zr = 0;               // |v| > support_zr: rotated template and mask are zero
     // captured scan (pf_scan_graph)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int64_t g_first = -1, g_count = -1;
-    int g_batch = -1, g_mode = -1, g_cz = -1, g_cl = -1;
+    int g_batch = -1, g_mode = -1, g_cz = -1, g_cl = -1, g_zs = -1;
     int num_cu = 0;
     double* d_rot = nullptr;
     int64_t nrot = 0;
@@ -476,6 +477,7 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     fa_args.occ = c->d_occ;
     fa_args.rmax = c->rmax;
     fa_args.zr = c->support_zr;
+    fa_args.zsched = c->zsched;
     fa_args.skip_planes = fa && fb;
     fa_args.work = c->d_cplx;
     fa_args.F1 = c->d_F1;
@@ -587,6 +589,13 @@ int pf_ctx_set_tuning(pf_ctx* c, int chunk_z, int chunk_lcc) {
     return kOk;
 }
 
+int pf_ctx_set_schedule(pf_ctx* c, int z_schedule) {
+    if (!c) return arg_error("ctx is null");
+    if (z_schedule < 0 || z_schedule >= PF_Z_SCHEDULES) return arg_error("unknown z schedule");
+    c->zsched = z_schedule;
+    return kOk;
+}
+
 int pf_scan_graph(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     if (!c) return arg_error("ctx is null");
     if (c->profile) {
@@ -596,7 +605,7 @@ int pf_scan_graph(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) 
     PF_DEVICE_GUARD(c);
     const bool same = c->graph_exec && c->g_first == first && c->g_count == count &&
                       c->g_batch == batch && c->g_mode == mode && c->g_cz == c->chunk_z &&
-                      c->g_cl == c->chunk_lcc;
+                      c->g_cl == c->chunk_lcc && c->g_zs == c->zsched;
     if (!same) {
         if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
         if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -624,6 +633,7 @@ int pf_scan_graph(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) 
         c->g_mode = mode;
         c->g_cz = c->chunk_z;
         c->g_cl = c->chunk_lcc;
+        c->g_zs = c->zsched;
     }
     PF_HIP_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
     return kOk;

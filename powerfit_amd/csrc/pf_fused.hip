@@ -920,7 +920,23 @@ void launch_fwd_pair(dim3 grid, hipStream_t s, const float2* tm, const double* r
                            zlo, out, twM, twN, occ);
 }
 
-template <int N, int NTHR1, int TQ, int EMAXZ, int NTHR3, int EMAX = 16>
+// z stage with element bound EZ (its radix schedule: pick_T(N, EZ, 32)).
+template <int N, int TQ, int EZ>
+void launch_zcorr(const FusedArgs& a, int ch, int zr_read, const float2* twN, hipStream_t s) {
+    using G = Geo<N>;
+    const dim3 grid((G::PLANE + TQ - 1) / TQ, (a.nrot + ch - 1) / ch);
+    const int nthr = TQ * pick_T(N, EZ, 32);
+    if (a.binary_mask)
+        hipLaunchKernelGGL((k_zcorr<N, TQ, EZ, true>), grid, dim3(nthr), 0, s, a.work, a.F1, a.F2,
+                           a.nrot, ch, zr_read, twN);
+    else
+        hipLaunchKernelGGL((k_zcorr<N, TQ, EZ, false>), grid, dim3(nthr), 0, s, a.work, a.F1,
+                           a.F2, a.nrot, ch, zr_read, twN);
+}
+
+// EMAXZ / EMAXZ2: element bounds of the default and the alternative z-FFT
+// schedules (FusedArgs::zsched picks one).
+template <int N, int NTHR1, int TQ, int EMAXZ, int NTHR3, int EMAX = 16, int EMAXZ2 = EMAXZ>
 int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     using G = Geo<N>;
     Tw tw;
@@ -947,14 +963,10 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     }
     if (which < 0 || which == 1) {
         const int ch = a.chunk_z;
-        const dim3 grid((G::PLANE + TQ - 1) / TQ, (a.nrot + ch - 1) / ch);
-        const int nthr = TQ * pick_T(N, EMAXZ, 32);
-        if (a.binary_mask)
-            hipLaunchKernelGGL((k_zcorr<N, TQ, EMAXZ, true>), grid, dim3(nthr), 0, s, a.work,
-                               a.F1, a.F2, a.nrot, ch, zr_read, tw.N);
+        if (a.zsched == 1)
+            launch_zcorr<N, TQ, EMAXZ2>(a, ch, zr_read, tw.N, s);
         else
-            hipLaunchKernelGGL((k_zcorr<N, TQ, EMAXZ, false>), grid, dim3(nthr), 0, s, a.work,
-                               a.F1, a.F2, a.nrot, ch, zr_read, tw.N);
+            launch_zcorr<N, TQ, EMAXZ>(a, ch, zr_read, tw.N, s);
         PF_CHECK_STATUS(last("k_zcorr"));
     }
     if (which < 0 || which == 2) {
@@ -994,12 +1006,12 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         return kErrArg;
     }
     switch (a.n) {
-        case 48: return run_fused<48, 192, 32, 12, 192>(a, which, s);
-        case 64: return run_fused<64, 256, 32, 8, 256>(a, which, s);
-        case 72: return run_fused<72, 384, 32, 12, 384>(a, which, s);
-        case 80: return run_fused<80, 320, 32, 16, 320, 12>(a, which, s);
-        case 96: return run_fused<96, 384, 32, 12, 384>(a, which, s);
-        case 128: return run_fused<128, 1024, 32, 16, 1024>(a, which, s);
+        case 48: return run_fused<48, 192, 32, 12, 192, 16, 6>(a, which, s);
+        case 64: return run_fused<64, 256, 32, 8, 256, 16, 4>(a, which, s);
+        case 72: return run_fused<72, 384, 32, 12, 384, 16, 6>(a, which, s);
+        case 80: return run_fused<80, 320, 32, 16, 320, 12, 24>(a, which, s);
+        case 96: return run_fused<96, 384, 32, 12, 384, 16, 24>(a, which, s);
+        case 128: return run_fused<128, 1024, 32, 16, 1024, 16, 8>(a, which, s);
         default:
             set_error("fused pipeline not built for n = %d", a.n);
             return kErrUnsupported;

@@ -317,3 +317,32 @@ def test_fused_serving_sizes_vs_oracle(n):
     corr.scan()
     err, dec = assert_scan_parity(corr.lcc, corr.rot, ref)
     say(f"  max err {err:.2e}, decisive voxels {dec}")
+
+
+SCHEDULE_CASES = [("f48", 48), ("cpu64", 64), ("f72", 72), ("f80", 80), ("lc96", 96),
+                  ("search128", 128)]
+
+
+@pytest.mark.parametrize("name,n", SCHEDULE_CASES)
+def test_alternative_z_schedule_vs_oracle(name, n):
+    """The alternative radix schedule of the fused z stage at every fused size:
+    oracle parity, and agreement with the default schedule to FP32 rounding."""
+    w = {48: NEW_SIZES[48], 72: NEW_SIZES[72], 80: NEW_SIZES[80]}.get(n) or WORKLOADS[name]
+    c = make_case(w)
+    R = rotations_for_angle(w.angle)[:24]
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
+                           laplace=c.laplace, nproc=NPROC)
+    say(f"scan {n}^3 fused, z schedules 0 and 1, 24 rotations")
+    outs = []
+    for zs in (0, 1):
+        corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused",
+                             batch=8)
+        corr.set_schedule(zs)
+        corr.template = c.template
+        corr.mask = c.mask
+        corr.rotations = R
+        corr.scan()
+        err, dec = assert_scan_parity(corr.lcc, corr.rot, ref)
+        say(f"  schedule {zs}: max err {err:.2e}, decisive voxels {dec}")
+        outs.append(corr.lcc.copy())
+    assert np.abs(outs[0] - outs[1]).max() < 1e-5
