@@ -990,7 +990,8 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
 bool fused_supported(int nz, int ny, int nx) {
     if (nz != ny || ny != nx) return false;
     switch (nx) {
-        case 48: case 64: case 72: case 80: case 96: case 128: return true;
+        case 48: case 64: case 72: case 80: case 96: case 128: case 144: case 160: case 192:
+            return true;
         default: return false;
     }
 }
@@ -1012,6 +1013,9 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         case 80: return run_fused<80, 320, 32, 16, 320, 12, 24>(a, which, s);
         case 96: return run_fused<96, 384, 32, 12, 384, 16, 24>(a, which, s);
         case 128: return run_fused<128, 1024, 32, 16, 1024, 16, 8>(a, which, s);
+        case 144: return run_fused<144, 864, 32, 12, 864, 16, 6>(a, which, s);
+        case 160: return run_fused<160, 704, 32, 10, 704, 20, 20>(a, which, s);
+        case 192: return run_fused<192, 784, 32, 12, 784, 24, 24>(a, which, s);
         default:
             set_error("fused pipeline not built for n = %d", a.n);
             return kErrUnsupported;

@@ -298,6 +298,9 @@ NEW_SIZES = {
     48: Workload("f48", 48, 2.0, 600, 8.0, 20.0, seed=7500),
     72: Workload("f72", 72, 1.8, 2000, 7.0, 20.0, laplace=True, core_weighted=True, seed=7600),
     80: Workload("f80", 80, 1.5, 2500, 6.0, 20.0, seed=7700),
+    144: Workload("f144", 144, 1.2, 6000, 5.0, 20.0, laplace=True, seed=7800),
+    160: Workload("f160", 160, 1.2, 8000, 5.0, 20.0, seed=7900),
+    192: Workload("f192", 192, 1.0, 10000, 5.0, 20.0, core_weighted=True, seed=8000),
 }
 
 
@@ -320,14 +323,14 @@ def test_fused_serving_sizes_vs_oracle(n):
 
 
 SCHEDULE_CASES = [("f48", 48), ("cpu64", 64), ("f72", 72), ("f80", 80), ("lc96", 96),
-                  ("search128", 128)]
+                  ("search128", 128), ("f144", 144), ("f160", 160), ("f192", 192)]
 
 
 @pytest.mark.parametrize("name,n", SCHEDULE_CASES)
 def test_alternative_z_schedule_vs_oracle(name, n):
     """The alternative radix schedule of the fused z stage at every fused size:
     oracle parity, and agreement with the default schedule to FP32 rounding."""
-    w = {48: NEW_SIZES[48], 72: NEW_SIZES[72], 80: NEW_SIZES[80]}.get(n) or WORKLOADS[name]
+    w = NEW_SIZES[n] if n in NEW_SIZES else WORKLOADS[name]
     c = make_case(w)
     R = rotations_for_angle(w.angle)[:24]
     ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
