@@ -395,8 +395,16 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
 }
 
 // ---------------------------------------------------------------------------
-template <int N, int NTHR, int EMAX>
-__global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X,
+// XPF (one-sweep column plans only): the next plane is loaded while the current
+// one is transformed (costs EY registers through the row phase); otherwise a
+// plane's first sweep is loaded when the plane starts and each later sweep
+// while the previous one is transformed.  ISDG: B, then
+// 1/sqrt(N B - A^2), of each owned voxel is kept in global memory instead of
+// registers, in the B plane's own region of X (this block is its only reader,
+// and the plane has been read in full before the first value is written);
+// each thread writes and reads back its own pairs, so no barrier is needed.
+template <int N, int NTHR, int EMAX, int EMAXC = EMAX, bool XPF = true, bool ISDG = false>
+__global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<ISDG, float2*, const float2*>::type __restrict__ X,
                                                     const uint8_t* __restrict__ lmask,
                                                     unsigned long long* __restrict__ keys,
                                                     int nrot, int chunk, int64_t first,
@@ -409,14 +417,18 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
     constexpr int EX = M / TX;
     constexpr int RPI = NTHR / TX;
     constexpr int ITERS = (N + RPI - 1) / RPI;
-    constexpr int TY = pick_T_cols(N, M, NTHR, EMAX);
+    // the column pass runs in CSW sweeps of CPI columns (fewer registers per
+    // thread than one sweep over all M columns)
+    constexpr int TY = pick_T_cols(N, M, NTHR, EMAXC);
     constexpr int EY = N / TY;
     constexpr int CPI = NTHR / TY;
+    constexpr int CSW = (M + CPI - 1) / CPI;
     constexpr bool WROWS = kWaveRows && (64 % TX == 0);
     constexpr bool WCOLS = kWaveCols && (64 % TY == 0);
-    static_assert(TY > 0 && NTHR % TY == 0 && CPI >= M, "one column pass");
+    static_assert(TY > 0 && NTHR % TY == 0, "column plan");
     static_assert(NTHR >= N, "one Nyquist element per thread");
     static_assert(ITERS * EX * 2 <= 64, "owned voxels per thread must fit a 64-bit mask");
+    static_assert(N * M <= G::PLANE, "the B plane's region holds one float2 per voxel pair");
     __shared__ float2 plane[N * G::S];
     __shared__ float2 colM[N];
     __shared__ float2 stwM[M], stwN[N];
@@ -430,13 +442,13 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
     // column pass: lanes run across columns (coalesced loads, conflict-free LDS)
     const int cg = WCOLS ? threadIdx.x / TY : threadIdx.x % CPI;
     const int ty = WCOLS ? threadIdx.x % TY : threadIdx.x / CPI;
-    const bool cact = cg < M;
     const float invVf = (float)invV;
 
     // owned voxels: (row = it*RPI + gx, x = 2 (tx + TX e) + h), bit 2(it*EX+e)+h
     using MaskT = typename std::conditional<(ITERS * EX * 2 <= 32), uint32_t, uint64_t>::type;
     MaskT mbits = 0;
-    float isd[ITERS][EX][2];
+    constexpr int IR = ISDG ? 1 : ITERS, ER = ISDG ? 1 : EX;
+    float isd[IR][ER][2];
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const int row = it * RPI + gx;
@@ -452,38 +464,66 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
 
     // Planes are visited in the order (b, B), (b, A), (b, G), (b + 1, B), ...
     // Each plane goes from global memory straight into the column registers
-    // (column cg, rows ty + TY e; the Nyquist column kx = M, one element per
-    // thread, through a small LDS buffer); the next plane's loads are issued
-    // before the current one is transformed, so their latency overlaps the FFTs.
+    // (sweep s: column cg + s CPI, rows ty + TY e; the Nyquist column kx = M,
+    // one element per thread, through a small LDS buffer); loads are issued
+    // ahead of the transform that needs them (see XPF above).
     const int nplanes = 3 * (b1 - b0);
     auto src_of = [&](int p) {
         const int b = b0 + p / 3, g = 2 - p % 3;
         return X + ((int64_t)b * 3 + g) * G::H + (int64_t)iz * G::PLANE;
     };
+    constexpr bool ONE = CSW == 1 && XPF;   // one sweep, next plane prefetched
     float2 pf[EY];
     float2 pm = c2(0.f, 0.f);
-    auto prefetch = [&](const float2* src) {
-        if (cact) {
+    const bool cact0 = cg < M;
+    auto prefetch = [&](const float2* src) {   // the ONE form
+        if (cact0) {
 #pragma unroll
             for (int e = 0; e < EY; ++e) pf[e] = src[(ty + TY * e) * G::NXH + cg];
         }
         if (threadIdx.x < N) pm = src[threadIdx.x * G::NXH + M];
     };
-    if (nplanes > 0) prefetch(src_of(0));
+    auto load_sweep = [&](const float2* src, int sw) {
+        const int col = cg + sw * CPI;
+        if (col < M) {
+#pragma unroll
+            for (int e = 0; e < EY; ++e) pf[e] = src[(ty + TY * e) * G::NXH + col];
+        }
+        if (sw == 0 && threadIdx.x < N) pm = src[threadIdx.x * G::NXH + M];
+    };
+    if (ONE && nplanes > 0) prefetch(src_of(0));
     for (int p = 0; p < nplanes; ++p) {
         const int b = b0 + p / 3, g = 2 - p % 3;
         __syncthreads();   // previous plane's row phase is done with the LDS
+        if (!ONE) load_sweep(src_of(p), 0);
         if (threadIdx.x < N) colM[threadIdx.x] = pm;
         __syncthreads();
         // y inverse of the M packed columns: column 0 carries X0 + i XM
-        float2 v[EY];
+        if constexpr (ONE) {
+            float2 v[EY];
 #pragma unroll
-        for (int e = 0; e < EY; ++e) {
-            v[e] = pf[e];
-            if (cg == 0) v[e] = cadd(v[e], mul_si<1>(colM[ty + TY * e]));
+            for (int e = 0; e < EY; ++e) {
+                v[e] = pf[e];
+                if (cg == 0) v[e] = cadd(v[e], mul_si<1>(colM[ty + TY * e]));
+            }
+            if (p + 1 < nplanes) prefetch(src_of(p + 1));
+            group_fft_ex<N, TY, 1, WCOLS>(v, plane + (cact0 ? cg : 0), G::S, ty, stwN, cact0);
+        } else {
+#pragma unroll
+            for (int sw = 0; sw < CSW; ++sw) {
+                const int col = cg + sw * CPI;
+                const bool cact = col < M;
+                float2 v[EY];
+#pragma unroll
+                for (int e = 0; e < EY; ++e) {
+                    v[e] = pf[e];
+                    if (col == 0) v[e] = cadd(v[e], mul_si<1>(colM[ty + TY * e]));
+                }
+                if (sw + 1 < CSW) load_sweep(src_of(p), sw + 1);
+                group_fft_ex<N, TY, 1, WCOLS>(v, plane + (cact ? col : 0), G::S, ty, stwN,
+                                              cact);
+            }
         }
-        if (p + 1 < nplanes) prefetch(src_of(p + 1));
-        group_fft_ex<N, TY, 1, WCOLS>(v, plane + (cact ? cg : 0), G::S, ty, stwN, cact);
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITERS; ++it) {
@@ -506,22 +546,51 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(const float2* __restrict__ X
             // the row's own threads read it above and overwrite it below
             group_sync<WROWS>();
             group_fft<M, TX, 1, WROWS>(vr, rowp, 1, tx, stwM, active);
-            if (active) {
+            if constexpr (ISDG) {
+                if (!active) continue;
+                // B, then 1/sqrt(var), of voxels x = 2m, 2m + 1 in the B plane's region
+                float2* slot = const_cast<float2*>(X) + ((int64_t)b * 3 + 2) * G::H +
+                               (int64_t)iz * G::PLANE + row * M + tx;
+#pragma unroll
+                for (int e = 0; e < EX; ++e) {
+                    float2 held = g == 2 ? vr[e] : slot[TX * e];
+                    if (g == 1) {
+                        // N<W^2 T^2> - <W T>^2 in FP64, then 1/sqrt in FP32
+                        const double ax = (double)vr[e].x * invV, ay = (double)vr[e].y * invV;
+                        held.x = rsqrtf((float)((double)held.x * invV * norm - ax * ax));
+                        held.y = rsqrtf((float)((double)held.y * invV * norm - ay * ay));
+                    } else if (g == 0) {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            if (!((mbits >> (2 * (it * EX + e) + h)) & MaskT(1))) continue;
+                            // only S > 0 can beat the initial (0, 0); NaN never wins
+                            const float sv = (h ? vr[e].y : vr[e].x) * invVf * (h ? held.y : held.x);
+                            if (sv > 0.f) {
+                                const int64_t vox =
+                                    ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
+                                atomicMax(keys + vox, lcc_key(sv, (uint32_t)(first + b)));
+                            }
+                        }
+                    }
+                    if (g != 0) slot[TX * e] = held;
+                }
+            } else if (active) {
 #pragma unroll
                 for (int e = 0; e < EX; ++e) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         const float val = h ? vr[e].y : vr[e].x;
+                        float& is = isd[ISDG ? 0 : it][ISDG ? 0 : e][h];
                         if (g == 2) {
-                            isd[it][e][h] = val;
+                            is = val;
                         } else if (g == 1) {
                             // N<W^2 T^2> - <W T>^2 in FP64, then 1/sqrt in FP32
                             const double a = (double)val * invV;
-                            const double a2 = (double)isd[it][e][h] * invV * norm;
-                            isd[it][e][h] = rsqrtf((float)(a2 - a * a));
+                            const double a2 = (double)is * invV * norm;
+                            is = rsqrtf((float)(a2 - a * a));
                         } else if ((mbits >> (2 * (it * EX + e) + h)) & MaskT(1)) {
                             // only S > 0 can beat the initial (0, 0); NaN never wins
-                            const float sv = val * invVf * isd[it][e][h];
+                            const float sv = val * invVf * is;
                             if (sv > 0.f) {
                                 const int64_t vox =
                                     ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
@@ -935,8 +1004,10 @@ void launch_zcorr(const FusedArgs& a, int ch, int zr_read, const float2* twN, hi
 }
 
 // EMAXZ / EMAXZ2: element bounds of the default and the alternative z-FFT
-// schedules (FusedArgs::zsched picks one).
-template <int N, int NTHR1, int TQ, int EMAXZ, int NTHR3, int EMAX = 16, int EMAXZ2 = EMAXZ>
+// schedules (FusedArgs::zsched picks one).  EMAX3 / EMAXC3 / XPF3 / ISDG3: the
+// one-plane inverse kernel's row and column element bounds and its options.
+template <int N, int NTHR1, int TQ, int EMAXZ, int NTHR3, int EMAX = 16, int EMAXZ2 = EMAXZ,
+          int EMAX3 = EMAX, int EMAXC3 = EMAX3, bool XPF3 = true, bool ISDG3 = false>
 int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     using G = Geo<N>;
     Tw tw;
@@ -977,7 +1048,8 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
                                dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
                                a.norm, a.invV, tw.M, tw.N);
         else
-            hipLaunchKernelGGL((k_inv_plane<N, NTHR3, EMAX>), dim3(N, (a.nrot + ch - 1) / ch),
+            hipLaunchKernelGGL((k_inv_plane<N, NTHR3, EMAX3, EMAXC3, XPF3, ISDG3>),
+                               dim3(N, (a.nrot + ch - 1) / ch),
                                dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
                                a.norm, a.invV, tw.M, tw.N);
         PF_CHECK_STATUS(last("k_inv_plane"));
@@ -1014,8 +1086,8 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         case 96: return run_fused<96, 384, 32, 12, 384, 16, 24>(a, which, s);
         case 128: return run_fused<128, 1024, 32, 16, 1024, 16, 8>(a, which, s);
         case 144: return run_fused<144, 864, 32, 12, 864, 16, 6>(a, which, s);
-        case 160: return run_fused<160, 704, 32, 10, 704, 20, 20>(a, which, s);
-        case 192: return run_fused<192, 784, 32, 12, 784, 24, 24>(a, which, s);
+        case 160: return run_fused<160, 704, 32, 10, 640, 20, 20, 10, 20, false, true>(a, which, s);
+        case 192: return run_fused<192, 784, 32, 12, 768, 24, 24, 12, 12, false, true>(a, which, s);
         default:
             set_error("fused pipeline not built for n = %d", a.n);
             return kErrUnsupported;
