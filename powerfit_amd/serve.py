@@ -20,6 +20,7 @@ order-independent max, so a result is identical alone and in the mix.
 prints one JSON line with throughput and p50/p99 latency.
 """
 import argparse
+import ctypes
 import json
 import time
 from dataclasses import dataclass, field
@@ -93,6 +94,15 @@ class FitServer:
         self.waiting = []
         self.results = {}
 
+    def warmup(self):
+        """Load the library and start the HIP runtime on the device (a
+        long-running server does this once, before its first request)."""
+        lib = _lib.load()
+        count = ctypes.c_int(0)
+        _lib.check(lib.pf_device_count(ctypes.byref(count)), "pf_device_count")
+        if self.device >= count.value:
+            raise RuntimeError(f"device {self.device} not present ({count.value} visible)")
+
     def _admit(self, now):
         while self.waiting and len(self.active) < self.max_active:
             req = self.waiting[0]
@@ -144,6 +154,7 @@ class FitServer:
         """Serve `requests` (arrival times in seconds from the start when
         realtime, else all at once).  Returns {rid: FitResult}."""
         pending = sorted(requests, key=lambda r: r.arrival)
+        self.warmup()
         t0 = time.perf_counter()
 
         def now():
@@ -197,6 +208,8 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--max-active", type=int, default=8)
     ap.add_argument("--budget-gib", type=float, default=64.0)
+    ap.add_argument("--detail", action="store_true",
+                    help="also print one JSON line per request (size, rotations, times)")
     args = ap.parse_args(argv)
     reqs = make_stream(args.requests, args.rate, args.seed)
     server = FitServer(hbm_budget=int(args.budget_gib * (1 << 30)), max_active=args.max_active,
@@ -204,6 +217,12 @@ def main(argv=None):
     t = time.perf_counter()
     res = server.run(reqs)
     wall = time.perf_counter() - t
+    if args.detail:
+        for r in sorted(res.values(), key=lambda r: r.rid):
+            print(json.dumps({"request": r.rid, "n": r.shape[0], "rotations": r.rotations,
+                              "arrival_s": round(r.arrival, 4),
+                              "queued_s": round(r.admitted - r.arrival, 4),
+                              "latency_s": round(r.latency, 4)}))
     lat = np.array([r.latency for r in res.values()])
     rots = sum(r.rotations for r in res.values())
     print(json.dumps({
