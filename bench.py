@@ -97,15 +97,17 @@ def cpu_baseline(case, R, nproc, per_worker):
     return done, secs
 
 
-def cpu_baseline_check(case, Rs, lcc, rot, nproc=1):
-    """Oracle result on rotations Rs and the GPU (lcc, rot) compared with it."""
-    from oracle import correlator as oc
-    if nproc > 1:
-        ref = oc.parallel_scan(case.target.astype(np.float64), case.template, case.mask, Rs,
-                               laplace=case.laplace, nproc=nproc)
-    else:
-        ref = oc.scan(case.target.astype(np.float64), case.template, case.mask, Rs,
-                      laplace=case.laplace)
+def cpu_baseline_check(case, Rs, lcc, rot, nproc=1, ref=None):
+    """Oracle result on rotations Rs (or `ref`, computed before) and the GPU
+    (lcc, rot) compared with it."""
+    if ref is None:
+        from oracle import correlator as oc
+        if nproc > 1:
+            ref = oc.parallel_scan(case.target.astype(np.float64), case.template, case.mask, Rs,
+                                   laplace=case.laplace, nproc=nproc)
+        else:
+            ref = oc.scan(case.target.astype(np.float64), case.template, case.mask, Rs,
+                          laplace=case.laplace)
     decisive = ref.margin() > 1e-4
     return {"max_abs_lcc_err": float(np.abs(lcc - ref.best).max()),
             "index_mismatches": int(((rot != ref.rot.astype(np.int64)) & decisive).sum()),
@@ -114,20 +116,27 @@ def cpu_baseline_check(case, Rs, lcc, rot, nproc=1):
 
 def precision_report(args, HipCorrelator, case, R, w):
     """FP32 vs split-precision (MFMA first stage) unfused passes on a rotation
-    prefix: rotations/s with inputs resident, and both results checked against
-    the FP64 oracle in the CPU leg."""
+    prefix, and the fused FP32 pipeline where the grid has one: rotations/s
+    with inputs resident, and every result checked against the FP64 oracle in
+    the CPU leg."""
     n = min(args.precision_rotations, R.shape[0])
     Rs = R[:n]
     rows = {}
     results = {}
-    for prec in ("fp32", "split"):
+    runs = [("fp32", "fp32", "batched"), ("split", "split", "batched")]
+    probe = HipCorrelator(case.target.astype(np.float64), laplace=case.laplace, device=0,
+                          mode="batched", batch=1)
+    if probe.fused_available:
+        runs.append(("fp32_fused", "fp32", "fused"))
+    del probe
+    for key, prec, mode in runs:
         corr = HipCorrelator(case.target.astype(np.float64), laplace=case.laplace, device=0,
-                             mode="batched", batch=args.precision_batch, precision=prec)
+                             mode=mode, batch=args.precision_batch, precision=prec)
         corr.template = case.template
         corr.mask = case.mask
         corr.rotations = Rs
         corr.scan()                      # warm-up, also the result checked below
-        results[prec] = (corr.lcc.copy(), corr.rot.copy())
+        results[key] = (corr.lcc.copy(), corr.rot.copy())
         best = float("inf")
         for _ in range(2):
             corr.reset()
@@ -136,16 +145,19 @@ def precision_report(args, HipCorrelator, case, R, w):
             corr.enqueue_scan(0, n)
             corr.sync()
             best = min(best, time.perf_counter() - t)
-        rows[prec] = {"rotations_per_s": n / best}
+        rows[key] = {"rotations_per_s": n / best, "path": f"{mode}, batch {args.precision_batch}"}
         del corr
+    from oracle import correlator as oc
     nproc = min(16, os.cpu_count() or 1)
-    for prec in ("fp32", "split"):
-        rows[prec].update(cpu_baseline_check(case, Rs, *results[prec], nproc=nproc))
+    ref = oc.parallel_scan(case.target.astype(np.float64), case.template, case.mask, Rs,
+                           laplace=case.laplace, nproc=nproc)
+    for key in rows:
+        rows[key].update(cpu_baseline_check(case, Rs, *results[key], ref=ref))
     nz, ny, nx = case.target.shape
-    return {"workload": f"{w.name}: {nz}x{ny}x{nx}, first {n} rotations of the {w.angle} deg set",
-            "path": "unfused batched passes, batch %d" % args.precision_batch,
-            "fp32": rows["fp32"], "split": rows["split"],
-            "split_vs_fp32_rate": rows["split"]["rotations_per_s"] / rows["fp32"]["rotations_per_s"]}
+    out = {"workload": f"{w.name}: {nz}x{ny}x{nx}, first {n} rotations of the {w.angle} deg set",
+           **rows,
+           "split_vs_fp32_rate": rows["split"]["rotations_per_s"] / rows["fp32"]["rotations_per_s"]}
+    return out
 
 
 def main(argv=None):
