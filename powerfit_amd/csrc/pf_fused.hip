@@ -201,7 +201,20 @@ __device__ __forceinline__ void gather_pw(const float2* __restrict__ tm, const u
 }
 
 // ---------------------------------------------------------------------------
-template <int N, int NTHR, bool BIN, int EMAX>
+// WSG: the mask values W of each thread's row voxels, gathered with P for grid 0
+// and reused for grids 1 and 2, are kept in global memory instead of registers,
+// in this plane's region of grid 2 (written last, by this block only, and
+// only after every read of it; each thread reads back only what it wrote).
+//
+// EMAXC: element bound of the column pass.  It runs in one sweep over the M
+// columns: the post-processing of column k reads columns k and M - k before
+// the FFT overwrites them.
+//
+// GONLY >= 0: this launch transforms grid GONLY only (the grids then run as
+// separate launches, grid 0 first; needs WSG).  A rolled loop over the grids
+// keeps every loop-invariant address live across its large body.
+template <int N, int NTHR, bool BIN, int EMAX, bool WSG = false, int EMAXC = EMAX,
+          int GONLY = -1>
 __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ tm,
                                                     const double* __restrict__ rots, int r,
                                                     int zr, int zlo,
@@ -216,7 +229,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     constexpr int RPI = NTHR / TX;
     constexpr int ITERS = (N + RPI - 1) / RPI;
     constexpr int OCCW = (G::NB * G::NB * G::NB + 31) / 32;
-    constexpr int TY = pick_T_cols(N, M, NTHR, EMAX);
+    constexpr int TY = pick_T_cols(N, M, NTHR, EMAXC);
     constexpr int EY = N / TY;
     constexpr int CPI = NTHR / TY;
     constexpr bool WROWS = kWaveRows && (64 % TX == 0);
@@ -243,13 +256,17 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     double R[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
-    float w[ITERS][EX][2];
+    float w[WSG ? 1 : ITERS][WSG ? 1 : EX][2];
     float2* dst = out + ((int64_t)b * 3) * G::H + (int64_t)iz * G::PLANE;
+    static_assert(N * M <= G::PLANE, "grid 2's region holds one float2 per voxel pair");
 
     // Grid g (0: P, 1: W, 2: W^2): gather the rows and run their half-length x
     // FFT, then the column pass (R2C post-processing, y FFT) and the store.
+    static_assert(GONLY < 0 || WSG, "separate grid launches share W through global memory");
+    constexpr int G0 = GONLY < 0 ? 0 : GONLY;
+    constexpr int G1 = GONLY < 0 ? (BIN ? 2 : 3) : GONLY + 1;
 #pragma unroll
-    for (int g = 0; g < (BIN ? 2 : 3); ++g) {
+    for (int g = G0; g < G1; ++g) {
 #pragma unroll
         for (int it = 0; it < ITERS; ++it) {
             const int row = it * RPI + gx;
@@ -263,7 +280,32 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                 for (int e = 0; e < EX; ++e) rowz[tx + TX * e] = c2(0.f, 0.f);
             }
             float2 v[EX];
-            if (active) {
+            if (active && WSG) {
+                float2* wst = dst + 2 * G::H + row * M + tx;
+#pragma unroll
+                for (int e = 0; e < EX; ++e) {
+                    const int m = tx + TX * e;
+                    if (g == 0) {
+                        float pv[2], wv[2];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int x = signed_coord<N>(2 * m + h, r);
+                            pv[h] = 0.f;
+                            wv[h] = 0.f;
+                            if (sz * sz + sy * sy + x * x <= r * r)
+                                gather_pw<N>(tm, occ, R, sz, sy, x, &pv[h], &wv[h]);
+                        }
+                        v[e] = c2(pv[0], pv[1]);
+                        wst[TX * e] = c2(wv[0], wv[1]);
+                        // keep the scheduler from hoisting every element's
+                        // gather loads at once (they exhaust the registers)
+                        __builtin_amdgcn_sched_barrier(0);
+                    } else {
+                        const float2 wv = wst[TX * e];
+                        v[e] = g == 1 ? wv : c2(wv.x * wv.x, wv.y * wv.y);
+                    }
+                }
+            } else if (active) {
 #pragma unroll
                 for (int e = 0; e < EX; ++e) {
                     const int m = tx + TX * e;
@@ -276,11 +318,11 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                             if (sz * sz + sy * sy + x * x <= r * r)
                                 gather_pw<N>(tm, occ, R, sz, sy, x, &p, &wv);
                             pv[h] = p;
-                            w[it][e][h] = wv;
+                            w[WSG ? 0 : it][WSG ? 0 : e][h] = wv;
                         } else if (g == 1) {
-                            pv[h] = w[it][e][h];
+                            pv[h] = w[WSG ? 0 : it][WSG ? 0 : e][h];
                         } else {
-                            pv[h] = w[it][e][h] * w[it][e][h];
+                            pv[h] = w[WSG ? 0 : it][WSG ? 0 : e][h] * w[WSG ? 0 : it][WSG ? 0 : e][h];
                         }
                     }
                     v[e] = c2(pv[0], pv[1]);
@@ -403,7 +445,13 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
 // registers, in the B plane's own region of X (this block is its only reader,
 // and the plane has been read in full before the first value is written);
 // each thread writes and reads back its own pairs, so no barrier is needed.
-template <int N, int NTHR, int EMAX, int EMAXC = EMAX, bool XPF = true, bool ISDG = false>
+//
+// RECOMP: the thread coordinates pass through an empty asm at each plane, so
+// the compiler recomputes the addresses derived from them there instead of
+// holding all of them in registers across the plane loop (fewer registers,
+// more address arithmetic; faster at some sizes only).
+template <int N, int NTHR, int EMAX, int EMAXC = EMAX, bool XPF = true, bool ISDG = false,
+          bool RECOMP = false>
 __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<ISDG, float2*, const float2*>::type __restrict__ X,
                                                     const uint8_t* __restrict__ lmask,
                                                     unsigned long long* __restrict__ keys,
@@ -438,33 +486,33 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
     const int iz = blockIdx.x;
     const int b0 = blockIdx.y * chunk;
     const int b1 = min(nrot, b0 + chunk);
-    const int gx = threadIdx.x / TX, tx = threadIdx.x % TX;
+    const int gx0 = threadIdx.x / TX, tx0 = threadIdx.x % TX;
     // column pass: lanes run across columns (coalesced loads, conflict-free LDS)
-    const int cg = WCOLS ? threadIdx.x / TY : threadIdx.x % CPI;
-    const int ty = WCOLS ? threadIdx.x % TY : threadIdx.x / CPI;
+    const int cg0 = WCOLS ? threadIdx.x / TY : threadIdx.x % CPI;
+    const int ty0 = WCOLS ? threadIdx.x % TY : threadIdx.x / CPI;
     const float invVf = (float)invV;
 
-    // owned voxels: (row = it*RPI + gx, x = 2 (tx + TX e) + h), bit 2(it*EX+e)+h
+    // owned voxels: (row = it*RPI + gx0, x = 2 (tx0 + TX e) + h), bit 2(it*EX+e)+h
     using MaskT = typename std::conditional<(ITERS * EX * 2 <= 32), uint32_t, uint64_t>::type;
     MaskT mbits = 0;
     constexpr int IR = ISDG ? 1 : ITERS, ER = ISDG ? 1 : EX;
     float isd[IR][ER][2];
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
-        const int row = it * RPI + gx;
+        const int row = it * RPI + gx0;
 #pragma unroll
         for (int e = 0; e < EX; ++e)
 #pragma unroll
             for (int h = 0; h < 2; ++h)
                 if (row < N) {
-                    const int64_t vox = ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
+                    const int64_t vox = ((int64_t)iz * N + row) * N + 2 * (tx0 + TX * e) + h;
                     if (lmask[vox]) mbits |= MaskT(1) << (2 * (it * EX + e) + h);
                 }
     }
 
     // Planes are visited in the order (b, B), (b, A), (b, G), (b + 1, B), ...
     // Each plane goes from global memory straight into the column registers
-    // (sweep s: column cg + s CPI, rows ty + TY e; the Nyquist column kx = M,
+    // (sweep s: column cg0 + s CPI, rows ty0 + TY e; the Nyquist column kx = M,
     // one element per thread, through a small LDS buffer); loads are issued
     // ahead of the transform that needs them (see XPF above).
     const int nplanes = 3 * (b1 - b0);
@@ -475,25 +523,28 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
     constexpr bool ONE = CSW == 1 && XPF;   // one sweep, next plane prefetched
     float2 pf[EY];
     float2 pm = c2(0.f, 0.f);
-    const bool cact0 = cg < M;
+    const bool cact0 = cg0 < M;
     auto prefetch = [&](const float2* src) {   // the ONE form
         if (cact0) {
 #pragma unroll
-            for (int e = 0; e < EY; ++e) pf[e] = src[(ty + TY * e) * G::NXH + cg];
+            for (int e = 0; e < EY; ++e) pf[e] = src[(ty0 + TY * e) * G::NXH + cg0];
         }
         if (threadIdx.x < N) pm = src[threadIdx.x * G::NXH + M];
     };
     auto load_sweep = [&](const float2* src, int sw) {
-        const int col = cg + sw * CPI;
+        const int col = cg0 + sw * CPI;
         if (col < M) {
 #pragma unroll
-            for (int e = 0; e < EY; ++e) pf[e] = src[(ty + TY * e) * G::NXH + col];
+            for (int e = 0; e < EY; ++e) pf[e] = src[(ty0 + TY * e) * G::NXH + col];
         }
         if (sw == 0 && threadIdx.x < N) pm = src[threadIdx.x * G::NXH + M];
     };
     if (ONE && nplanes > 0) prefetch(src_of(0));
     for (int p = 0; p < nplanes; ++p) {
         const int b = b0 + p / 3, g = 2 - p % 3;
+        int opq = 0;
+        if constexpr (RECOMP) asm volatile("" : "+v"(opq));
+        const int cg = cg0 + opq, ty = ty0 + opq, tx = tx0 + opq, gx = gx0 + opq;
         __syncthreads();   // previous plane's row phase is done with the LDS
         if (!ONE) load_sweep(src_of(p), 0);
         if (threadIdx.x < N) colM[threadIdx.x] = pm;
@@ -980,6 +1031,26 @@ int last(const char* what) {
     return kOk;
 }
 
+// K1 with one-plane passes: all grids in one launch, or (STASHW) one launch
+// per grid, grid 0 first.
+template <int N, int NTHR, bool BIN, int EMAX, bool STASHW, int EMAXC>
+void launch_fwd_plane(dim3 grid, hipStream_t s, const float2* tm, const double* rots, int r,
+                      int zr, int zlo, float2* out, const float2* twM, const float2* twN,
+                      const uint32_t* occ) {
+    if constexpr (STASHW) {
+        hipLaunchKernelGGL((k_fwd_plane<N, NTHR, BIN, EMAX, true, EMAXC, 0>), grid, dim3(NTHR),
+                           0, s, tm, rots, r, zr, zlo, out, twM, twN, occ);
+        hipLaunchKernelGGL((k_fwd_plane<N, NTHR, BIN, EMAX, true, EMAXC, 1>), grid, dim3(NTHR),
+                           0, s, tm, rots, r, zr, zlo, out, twM, twN, occ);
+        if constexpr (!BIN)
+            hipLaunchKernelGGL((k_fwd_plane<N, NTHR, BIN, EMAX, true, EMAXC, 2>), grid,
+                               dim3(NTHR), 0, s, tm, rots, r, zr, zlo, out, twM, twN, occ);
+    } else {
+        hipLaunchKernelGGL((k_fwd_plane<N, NTHR, BIN, EMAX, false, EMAXC>), grid, dim3(NTHR), 0,
+                           s, tm, rots, r, zr, zlo, out, twM, twN, occ);
+    }
+}
+
 template <int N, int NTHR, int EMAX>
 void launch_fwd_pair(dim3 grid, hipStream_t s, const float2* tm, const double* rots, int r, int zr,
                      int zlo, float2* out, const float2* twM, const float2* twN,
@@ -1003,12 +1074,42 @@ void launch_zcorr(const FusedArgs& a, int ch, int zr_read, const float2* twN, hi
                            a.F2, a.nrot, ch, zr_read, twN);
 }
 
-// EMAXZ / EMAXZ2: element bounds of the default and the alternative z-FFT
-// schedules (FusedArgs::zsched picks one).  EMAX3 / EMAXC3 / XPF3 / ISDG3: the
-// one-plane inverse kernel's row and column element bounds and its options.
-template <int N, int NTHR1, int TQ, int EMAXZ, int NTHR3, int EMAX = 16, int EMAXZ2 = EMAXZ,
-          int EMAX3 = EMAX, int EMAXC3 = EMAX3, bool XPF3 = true, bool ISDG3 = false>
+// Launch shape of one fused size.  K1: NTHR1 threads, element bounds EMAX (rows,
+// and the two-plane kernels) and EMAXC1 (one-plane column pass), mask values
+// kept in global memory and one launch per grid when STASHW.  K2: TQ columns per
+// block, element bounds EMAXZ / EMAXZ2 of the default and the alternative z
+// schedule (FusedArgs::zsched picks one).  K3: NTHR3 threads, element bounds
+// EMAX3 (rows) / EMAXC3 (columns), options XPF3, ISDG3 and RECOMP3 of the
+// one-plane kernel.  Sizes that differ from the defaults derive from Shape and
+// redefine members.
+template <int NTHR1_, int TQ_, int EZ_, int NTHR3_, int E_ = 16, int EZ2_ = EZ_>
+struct Shape {
+    static constexpr int NTHR1 = NTHR1_, TQ = TQ_, EMAXZ = EZ_, NTHR3 = NTHR3_, EMAX = E_,
+                         EMAXZ2 = EZ2_, EMAX3 = E_, EMAXC3 = E_, EMAXC1 = E_;
+    static constexpr bool XPF3 = true, ISDG3 = false, RECOMP3 = false, STASHW = false;
+};
+
+struct Shape80 : Shape<320, 32, 16, 320, 12, 24> {
+    static constexpr bool RECOMP3 = true;
+};
+struct Shape144 : Shape<864, 32, 12, 864, 16, 6> {
+    static constexpr bool RECOMP3 = true;
+};
+// 160 = 2^5 * 5 has no 16-element plan; K3 sweeps its 80 columns once with
+// 20 elements per thread and keeps the variance in global memory
+struct Shape160 : Shape<704, 32, 10, 640, 20, 20> {
+    static constexpr int EMAX3 = 10, EMAXC3 = 20;
+    static constexpr bool XPF3 = false, ISDG3 = true, STASHW = true;
+};
+// 192: K3 runs its 96 columns in two sweeps of 48 (12 elements per thread)
+struct Shape192 : Shape<784, 32, 12, 768, 24, 24> {
+    static constexpr int EMAX3 = 12, EMAXC3 = 12;
+    static constexpr bool XPF3 = false, ISDG3 = true, STASHW = true;
+};
+
+template <int N, class SH>
 int run_fused(const FusedArgs& a, int which, hipStream_t s) {
+    constexpr int NTHR1 = SH::NTHR1, TQ = SH::TQ, NTHR3 = SH::NTHR3, EMAX = SH::EMAX;
     using G = Geo<N>;
     Tw tw;
     PF_CHECK_STATUS(get_tw(N, &tw));
@@ -1023,21 +1124,21 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
             launch_fwd_pair<N, NTHR1, EMAX>(dim3(nplanes, a.nrot), s, a.tm, a.rots, a.rmax, a.zr,
                                             zlo, a.work, tw.M, tw.N, a.occ);
         else if (a.binary_mask)
-            hipLaunchKernelGGL((k_fwd_plane<N, NTHR1, true, EMAX>), dim3(nplanes, a.nrot),
-                               dim3(NTHR1), 0, s, a.tm, a.rots, a.rmax, a.zr, zlo, a.work, tw.M,
-                               tw.N, a.occ);
+            launch_fwd_plane<N, NTHR1, true, EMAX, SH::STASHW, SH::EMAXC1>(
+                dim3(nplanes, a.nrot), s, a.tm, a.rots, a.rmax, a.zr, zlo, a.work, tw.M, tw.N,
+                a.occ);
         else
-            hipLaunchKernelGGL((k_fwd_plane<N, NTHR1, false, EMAX>), dim3(nplanes, a.nrot),
-                               dim3(NTHR1), 0, s, a.tm, a.rots, a.rmax, a.zr, zlo, a.work, tw.M,
-                               tw.N, a.occ);
+            launch_fwd_plane<N, NTHR1, false, EMAX, SH::STASHW, SH::EMAXC1>(
+                dim3(nplanes, a.nrot), s, a.tm, a.rots, a.rmax, a.zr, zlo, a.work, tw.M, tw.N,
+                a.occ);
         PF_CHECK_STATUS(last("k_fwd_plane"));
     }
     if (which < 0 || which == 1) {
         const int ch = a.chunk_z;
         if (a.zsched == 1)
-            launch_zcorr<N, TQ, EMAXZ2>(a, ch, zr_read, tw.N, s);
+            launch_zcorr<N, TQ, SH::EMAXZ2>(a, ch, zr_read, tw.N, s);
         else
-            launch_zcorr<N, TQ, EMAXZ>(a, ch, zr_read, tw.N, s);
+            launch_zcorr<N, TQ, SH::EMAXZ>(a, ch, zr_read, tw.N, s);
         PF_CHECK_STATUS(last("k_zcorr"));
     }
     if (which < 0 || which == 2) {
@@ -1048,7 +1149,8 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
                                dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
                                a.norm, a.invV, tw.M, tw.N);
         else
-            hipLaunchKernelGGL((k_inv_plane<N, NTHR3, EMAX3, EMAXC3, XPF3, ISDG3>),
+            hipLaunchKernelGGL((k_inv_plane<N, NTHR3, SH::EMAX3, SH::EMAXC3, SH::XPF3, SH::ISDG3,
+                                            SH::RECOMP3>),
                                dim3(N, (a.nrot + ch - 1) / ch),
                                dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
                                a.norm, a.invV, tw.M, tw.N);
@@ -1079,15 +1181,15 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         return kErrArg;
     }
     switch (a.n) {
-        case 48: return run_fused<48, 192, 32, 12, 192, 16, 6>(a, which, s);
-        case 64: return run_fused<64, 256, 32, 8, 256, 16, 4>(a, which, s);
-        case 72: return run_fused<72, 384, 32, 12, 384, 16, 6>(a, which, s);
-        case 80: return run_fused<80, 320, 32, 16, 320, 12, 24>(a, which, s);
-        case 96: return run_fused<96, 384, 32, 12, 384, 16, 24>(a, which, s);
-        case 128: return run_fused<128, 1024, 32, 16, 1024, 16, 8>(a, which, s);
-        case 144: return run_fused<144, 864, 32, 12, 864, 16, 6>(a, which, s);
-        case 160: return run_fused<160, 704, 32, 10, 640, 20, 20, 10, 20, false, true>(a, which, s);
-        case 192: return run_fused<192, 784, 32, 12, 768, 24, 24, 12, 12, false, true>(a, which, s);
+        case 48: return run_fused<48, Shape<192, 32, 12, 192, 16, 6>>(a, which, s);
+        case 64: return run_fused<64, Shape<256, 32, 8, 256, 16, 4>>(a, which, s);
+        case 72: return run_fused<72, Shape<384, 32, 12, 384, 16, 6>>(a, which, s);
+        case 80: return run_fused<80, Shape80>(a, which, s);
+        case 96: return run_fused<96, Shape<384, 32, 12, 384, 16, 24>>(a, which, s);
+        case 128: return run_fused<128, Shape<1024, 32, 16, 1024, 16, 8>>(a, which, s);
+        case 144: return run_fused<144, Shape144>(a, which, s);
+        case 160: return run_fused<160, Shape160>(a, which, s);
+        case 192: return run_fused<192, Shape192>(a, which, s);
         default:
             set_error("fused pipeline not built for n = %d", a.n);
             return kErrUnsupported;
