@@ -1087,11 +1087,13 @@ struct Shape {
     static constexpr int NTHR1 = NTHR1_, TQ = TQ_, EMAXZ = EZ_, NTHR3 = NTHR3_, EMAX = E_,
                          EMAXZ2 = EZ2_, EMAX3 = E_, EMAXC3 = E_, EMAXC1 = E_;
     static constexpr bool XPF3 = true, ISDG3 = false, RECOMP3 = false, STASHW = false;
+    static constexpr bool PAIR3 = true;   // two-plane K3 where its plan fits
 };
 
 struct Shape80 : Shape<320, 32, 16, 320, 12, 24> {
     static constexpr bool RECOMP3 = true;
 };
+struct Shape128 : Shape<1024, 32, 16, 1024, 16, 8> {};
 struct Shape144 : Shape<864, 32, 12, 864, 16, 6> {
     static constexpr bool RECOMP3 = true;
 };
@@ -1144,7 +1146,7 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     if (which < 0 || which == 2) {
         const int ch = a.chunk_lcc;
         // two-slot inverse only for the large planes: it costs registers
-        if constexpr (PairPlan<N, NTHR3, EMAX>::ok && NTHR3 >= 512)
+        if constexpr (SH::PAIR3 && PairPlan<N, NTHR3, EMAX>::ok && NTHR3 >= 512)
             hipLaunchKernelGGL((k_inv_plane2<N, NTHR3, EMAX>), dim3(N, (a.nrot + ch - 1) / ch),
                                dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
                                a.norm, a.invV, tw.M, tw.N);
@@ -1186,7 +1188,7 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         case 72: return run_fused<72, Shape<384, 32, 12, 384, 16, 6>>(a, which, s);
         case 80: return run_fused<80, Shape80>(a, which, s);
         case 96: return run_fused<96, Shape<384, 32, 12, 384, 16, 24>>(a, which, s);
-        case 128: return run_fused<128, Shape<1024, 32, 16, 1024, 16, 8>>(a, which, s);
+        case 128: return run_fused<128, Shape128>(a, which, s);
         case 144: return run_fused<144, Shape144>(a, which, s);
         case 160: return run_fused<160, Shape160>(a, which, s);
         case 192: return run_fused<192, Shape192>(a, which, s);

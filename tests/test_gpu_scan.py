@@ -349,3 +349,54 @@ def test_alternative_z_schedule_vs_oracle(name, n):
         say(f"  schedule {zs}: max err {err:.2e}, decisive voxels {dec}")
         outs.append(corr.lcc.copy())
     assert np.abs(outs[0] - outs[1]).max() < 1e-5
+
+
+LARGE_MIX = [(160, "mix_fwd"), (160, "mix_inv"), (192, "mix_fwd"), (192, "mix_inv")]
+
+
+@pytest.mark.parametrize("n,mode", LARGE_MIX)
+def test_large_grid_stage_mix_vs_oracle(n, mode):
+    """The large grids' K1 (mask values parked in the work buffer, one launch
+    per grid) and K3 (variance parked in the consumed B plane, column sweeps)
+    next to the unfused stages, which read and write every grid themselves."""
+    w = NEW_SIZES[n]
+    c = make_case(w)
+    R = rotations_for_angle(w.angle)[:8]
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
+                           laplace=c.laplace, nproc=NPROC)
+    say(f"scan {n}^3 mode={mode}, 8 rotations")
+    best, rot = run_hip(c, R, mode, 4)
+    err, dec = assert_scan_parity(best, rot, ref)
+    say(f"  max err {err:.2e}, decisive voxels {dec}")
+
+
+@pytest.mark.parametrize("n", [160, 192])
+def test_large_grid_chunks_and_weighted_mask(n):
+    """Chunks of 1 and several rotations per K3 block, and the W^2 grid
+    transformed separately, give the same maps (chunks) and oracle parity."""
+    w = NEW_SIZES[n]
+    c = make_case(w)
+    R = rotations_for_angle(w.angle)[:12]
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
+                           laplace=c.laplace, nproc=NPROC)
+    say(f"scan {n}^3 fused, K3 chunks 1 and 5, W^2 separate")
+    outs = []
+    for cz, cl in ((1, 1), (2, 5)):
+        corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused",
+                             batch=12)
+        corr.set_tuning(cz, cl)
+        corr.template = c.template
+        corr.mask = c.mask
+        corr.rotations = R
+        corr.scan()
+        outs.append((corr.lcc.copy(), corr.rot.copy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert_scan_parity(outs[0][0], outs[0][1], ref)
+    corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused", batch=6,
+                         binary_shortcut=False)
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    corr.scan()
+    err, dec = assert_scan_parity(corr.lcc, corr.rot, ref)
+    say(f"  max err {err:.2e}, decisive voxels {dec}")
