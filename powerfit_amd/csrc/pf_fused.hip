@@ -14,9 +14,13 @@
 //                 registers for the whole chunk), inverse FFT along z.
 //   k_inv_plane   one block per (z-plane, rotation chunk): inverse C2C along
 //                 y and C2R along x with the plane in LDS, then
-//                 S = G / sqrt(N B - A^2) (variance in FP64) and a strict '>'
-//                 running max/argmax over the chunk in registers, merged into
-//                 the result with a 64-bit atomicMax on lcc_key.
+//                 S = G / sqrt(N B - A^2) (variance in FP64), merged per
+//                 rotation into the running max/argmax with a 64-bit
+//                 atomicMax on lcc_key (masked voxels with S > 0 only), which
+//                 equals the reference's strict '>' take-best in any order.
+//
+// k_fwd_plane2 / k_inv_plane2 are two-plane forms of the first and last
+// kernel; each size's launch shape is a Shape below.
 //
 // The DC and Nyquist columns of the x-transform are both real; they travel
 // through the y transform packed as one complex column X0 + i XM and are
@@ -65,45 +69,6 @@ __device__ __forceinline__ void stage_twiddles(float2* dst, const float2* __rest
     for (int i = threadIdx.x; i < L; i += NTHR) dst[i] = src[i];
 }
 
-// Column (y) FFT of the M packed columns of an LDS plane, NTHR threads.
-// WAVE: each column's threads are consecutive lanes of one wavefront and the
-// stages synchronise per wave; otherwise lanes run across columns (a wave
-// touches consecutive words) and stages use workgroup barriers.
-// PACK_IN: the plane holds the M+1 unpacked columns (DC at 0, Nyquist at M);
-// column 0 is transformed as DC + i Nyquist.
-template <int N, int NTHR, int DIR, bool WAVE, int EMAX, bool PACK_IN = false>
-__device__ __forceinline__ void plane_columns(float2* plane, const float2* __restrict__ twN) {
-    using G = Geo<N>;
-    constexpr int TY = pick_T_cols(N, G::M, NTHR, EMAX);
-    constexpr int EY = N / TY;
-    constexpr int CPI = NTHR / TY;
-    constexpr int CITERS = (G::M + CPI - 1) / CPI;
-    static_assert(TY > 0 && NTHR % TY == 0, "no column plan");
-    const int gy = WAVE ? threadIdx.x / TY : threadIdx.x % CPI;
-    const int ty = WAVE ? threadIdx.x % TY : threadIdx.x / CPI;
-#pragma unroll
-    for (int ci = 0; ci < CITERS; ++ci) {
-        const int c = ci * CPI + gy;
-        const bool active = (gy < CPI) && (c < G::M);
-        float2 v[EY];
-        if (active) {
-#pragma unroll
-            for (int e = 0; e < EY; ++e) {
-                v[e] = plane[(ty + TY * e) * G::S + c];
-                if (PACK_IN && c == 0)
-                    v[e] = cadd(v[e], mul_si<1>(plane[(ty + TY * e) * G::S + G::M]));
-            }
-        }
-        __syncthreads();
-        group_fft<N, TY, DIR, WAVE>(v, plane + (active ? c : 0), G::S, ty, twN, active);
-        if (active) {
-#pragma unroll
-            for (int e = 0; e < EY; ++e) plane[(ty + TY * e) * G::S + c] = v[e];
-        }
-        __syncthreads();
-    }
-}
-
 // Write the y-transformed plane to global [ky][kx], unpacking column 0.
 template <int N, int NTHR>
 __device__ __forceinline__ void store_plane(const float2* plane, float2* __restrict__ dst) {
@@ -119,23 +84,6 @@ __device__ __forceinline__ void store_plane(const float2* plane, float2* __restr
             val = plane[ky * G::S + kx];
         }
         dst[f] = val;
-    }
-}
-
-// Load a plane [ky][kx] from global into LDS, packing columns 0 and M.
-template <int N, int NTHR>
-__device__ __forceinline__ void load_plane(float2* plane, const float2* __restrict__ src) {
-    using G = Geo<N>;
-    for (int f = threadIdx.x; f < N * G::M; f += NTHR) {
-        const int ky = f / G::M, kx = f - ky * G::M;
-        float2 val;
-        if (kx == 0) {
-            const float2 y0 = src[ky * G::NXH], ym = src[ky * G::NXH + G::M];
-            val = cadd(y0, mul_si<1>(ym));
-        } else {
-            val = src[ky * G::NXH + kx];
-        }
-        plane[ky * G::S + kx] = val;
     }
 }
 
