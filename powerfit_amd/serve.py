@@ -9,7 +9,12 @@ the active fits batch by batch (each keeps at most ``depth`` batches queued on
 its stream), so work from several fits overlaps on the device and a request
 arriving mid-way starts as soon as it is admitted instead of waiting for the
 queue to drain.  A fit's result is read back when its stream is idle; its
-context is then released and the memory returned to the budget.
+context is then released and the memory returned to the budget — or, when
+the request names its map (``target_key``), kept resident with that map's
+spectra: a later request on the same map takes the idle context and only
+loads its template, mask and rotations.  Idle contexts count against the
+budget and are released, least recently used first, when a new fit needs the
+memory.
 
 Each fit's result is computed exactly as in a standalone search: its
 rotations go through the same kernels, and the per-voxel merge is an
@@ -23,6 +28,7 @@ import argparse
 import ctypes
 import json
 import time
+from collections import OrderedDict
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -41,6 +47,7 @@ class FitRequest:
     rotations: np.ndarray
     laplace: bool = False
     arrival: float = 0.0
+    target_key: object = None   # equal keys: the same map (its spectra can stay resident)
 
 
 @dataclass
@@ -82,8 +89,9 @@ class FitServer:
     """Admit, interleave and complete LCC searches on one device."""
 
     def __init__(self, device=0, hbm_budget=64 << 30, max_active=8, batch=64, depth=2,
-                 mode="auto"):
+                 mode="auto", keep_targets=True):
         self.device = device
+        self.keep_targets = keep_targets   # reuse idle contexts of requests' named maps
         self.hbm_budget = int(hbm_budget)
         self.max_active = max_active
         self.batch = batch
@@ -93,6 +101,8 @@ class FitServer:
         self.active = []
         self.waiting = []
         self.results = {}
+        self.idle = OrderedDict()   # (target_key, shape, laplace) -> [(corr, bytes)], LRU first
+        self.reused = 0
 
     def warmup(self):
         """Load the library and start the HIP runtime on the device (a
@@ -103,22 +113,57 @@ class FitServer:
         if self.device >= count.value:
             raise RuntimeError(f"device {self.device} not present ({count.value} visible)")
 
+    def _key(self, req):
+        if req.target_key is None or not self.keep_targets:
+            return None
+        return (req.target_key, tuple(req.target.shape), bool(req.laplace))
+
+    def _take_idle(self, key):
+        if key is None or key not in self.idle:
+            return None
+        ctxs = self.idle[key]
+        held = ctxs.pop()
+        if not ctxs:
+            del self.idle[key]
+        return held
+
+    def _evict_idle(self, need):
+        """Release idle contexts, least recently used first, until `need`
+        more bytes fit in the budget (or none is left)."""
+        while self.idle and self.used + need > self.hbm_budget:
+            key, ctxs = next(iter(self.idle.items()))
+            corr, nbytes = ctxs.pop(0)
+            if not ctxs:
+                del self.idle[key]
+            self.used -= nbytes
+            del corr
+
+    def release_idle(self):
+        """Release every idle context (their memory returns to the budget)."""
+        self._evict_idle(float("inf"))
+
     def _admit(self, now):
         while self.waiting and len(self.active) < self.max_active:
             req = self.waiting[0]
-            need = ctx_bytes(req.target.shape, self.batch, self.mode)
-            if need > self.hbm_budget:
-                raise MemoryError(f"request {req.rid} needs {need} bytes, over the budget")
-            if self.used + need > self.hbm_budget:
-                break
+            held = self._take_idle(self._key(req))
+            if held is not None:
+                corr, need = held           # its bytes are already counted
+                self.reused += 1
+            else:
+                need = ctx_bytes(req.target.shape, self.batch, self.mode)
+                if need > self.hbm_budget:
+                    raise MemoryError(f"request {req.rid} needs {need} bytes, over the budget")
+                self._evict_idle(need)
+                if self.used + need > self.hbm_budget:
+                    break
+                corr = HipCorrelator(np.asarray(req.target, np.float64), laplace=req.laplace,
+                                     device=self.device, mode=self.mode, batch=self.batch)
+                self.used += need
             self.waiting.pop(0)
-            corr = HipCorrelator(np.asarray(req.target, np.float64), laplace=req.laplace,
-                                 device=self.device, mode=self.mode, batch=self.batch)
             corr.template = req.template
             corr.mask = req.mask
             corr.rotations = req.rotations
             corr.reset()
-            self.used += need
             self.active.append(_Active(req, corr, need, now))
 
     def _pump(self):
@@ -144,8 +189,13 @@ class FitServer:
                 self.results[a.req.rid] = FitResult(
                     a.req.rid, lcc.copy(), rot.copy(), a.req.arrival, a.admitted, now_fn(),
                     a.req.rotations.shape[0], a.req.target.shape)
-                self.used -= a.bytes
-                del a.corr
+                key = self._key(a.req)
+                if key is None:
+                    self.used -= a.bytes
+                    del a.corr
+                else:                       # stays resident for the next fit on this map
+                    self.idle.setdefault(key, []).append((a.corr, a.bytes))
+                    self.idle.move_to_end(key)
             else:
                 still.append(a)
         self.active = still
@@ -178,25 +228,35 @@ class FitServer:
         return self.results
 
 
-def make_stream(n, rate, seed=5003, sizes=None, angles=(10.0, 20.0)):
+def make_stream(n, rate, seed=5003, sizes=None, angles=(10.0, 20.0), maps=0):
     """Poisson stream of synthetic fit requests (config: 500-20,000-atom chains
-    into 48^3-160^3 maps at 4-12 A, 10 or 20 degree rotation sets)."""
+    into 48^3-160^3 maps at 4-12 A, 10 or 20 degree rotation sets).  With
+    maps > 0 the requests draw from that many shared maps (target_key set), as
+    when several fits run against the same cryo-EM map."""
     from .rotations import rotations_for_angle
     from .synth import Workload, make_case
     rng = np.random.default_rng(seed)
     sizes = sizes or [48, 64, 72, 80, 96, 120, 128, 144, 160]
+
+    def draw_case(tag, k):
+        N = int(rng.choice(sizes))
+        res = float(rng.uniform(4.0, 12.0))
+        atoms = int(rng.integers(500, 20001))
+        return make_case(Workload(tag, N, res / 4.0, atoms, res, 10.0, seed=seed * 10 + k * 7))
+
+    shared = [draw_case(f"map{j}", 100000 + j) for j in range(maps)]
     t = 0.0
     reqs = []
     for i in range(n):
         t += rng.exponential(1.0 / rate)
-        N = int(rng.choice(sizes))
-        res = float(rng.uniform(4.0, 12.0))
-        atoms = int(rng.integers(500, 20001))
+        if maps:
+            j = int(rng.integers(0, maps))
+            c, key = shared[j], f"map{j}"
+        else:
+            c, key = draw_case(f"req{i}", i), None
         angle = float(rng.choice(angles))
-        w = Workload(f"req{i}", N, res / 4.0, atoms, res, angle, seed=seed * 10 + i * 7)
-        c = make_case(w)
         reqs.append(FitRequest(i, c.target, c.template, c.mask, rotations_for_angle(angle),
-                               c.laplace, t))
+                               c.laplace, t, key))
     return reqs
 
 
@@ -208,12 +268,16 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--max-active", type=int, default=8)
     ap.add_argument("--budget-gib", type=float, default=64.0)
+    ap.add_argument("--maps", type=int, default=0,
+                    help="draw the requests from this many shared maps (0: one map per request)")
+    ap.add_argument("--no-keep-targets", action="store_true",
+                    help="release every context after its fit (no resident map spectra)")
     ap.add_argument("--detail", action="store_true",
                     help="also print one JSON line per request (size, rotations, times)")
     args = ap.parse_args(argv)
-    reqs = make_stream(args.requests, args.rate, args.seed)
+    reqs = make_stream(args.requests, args.rate, args.seed, maps=args.maps)
     server = FitServer(hbm_budget=int(args.budget_gib * (1 << 30)), max_active=args.max_active,
-                       batch=args.batch)
+                       batch=args.batch, keep_targets=not args.no_keep_targets)
     t = time.perf_counter()
     res = server.run(reqs)
     wall = time.perf_counter() - t
@@ -229,7 +293,8 @@ def main(argv=None):
         "requests": len(res), "wall_s": wall, "rotations": rots,
         "rotations_per_s": rots / wall, "requests_per_s": len(res) / wall,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
-        "sizes": sorted({r.shape[0] for r in res.values()}), "arrival_rate": args.rate}))
+        "sizes": sorted({r.shape[0] for r in res.values()}), "arrival_rate": args.rate,
+        "maps": args.maps, "contexts_reused": server.reused}))
 
 
 if __name__ == "__main__":

@@ -273,6 +273,38 @@ def test_serving_results_equal_standalone():
         assert np.array_equal(res[rid].lcc, lcc) and np.array_equal(res[rid].rot, rot), rid
 
 
+def test_serving_keeps_target_spectra_resident():
+    """Requests on the same map reuse its idle context (spectra stay in HBM),
+    with results identical to standalone fits; idle contexts stay counted
+    against the budget until released."""
+    from powerfit_amd.serve import FitRequest, FitServer
+    say("serving: four fits on two maps, contexts reused")
+    maps = []
+    for name in ("cpu64", "lc96"):
+        w = SMALL.get(name) or WORKLOADS[name]
+        maps.append((make_case(w), rotations_for_angle(w.angle)))
+    reqs = []
+    for i in range(4):
+        cs, R = maps[i % 2]
+        Ri = R[:24] if i < 2 else R[24:56]
+        reqs.append(FitRequest(i, cs.target, cs.template, cs.mask, Ri, cs.laplace, 0.0,
+                               f"map{i % 2}"))
+    server = FitServer(max_active=1, batch=8, depth=2)
+    res = server.run(reqs, realtime=False)
+    assert sorted(res) == [0, 1, 2, 3] and server.reused == 2
+    assert server.used > 0 and len(server.idle) == 2
+    for r in reqs:
+        corr = HipCorrelator(r.target.astype(np.float64), laplace=r.laplace, batch=8)
+        corr.template = r.template
+        corr.mask = r.mask
+        corr.rotations = r.rotations
+        corr.scan()
+        assert np.array_equal(res[r.rid].lcc, corr.lcc), r.rid
+        assert np.array_equal(res[r.rid].rot, corr.rot), r.rid
+    server.release_idle()
+    assert server.used == 0 and not server.idle
+
+
 def test_serving_admission_respects_budget():
     from powerfit_amd.serve import FitRequest, FitServer, ctx_bytes
     w = WORKLOADS["cpu64"]
