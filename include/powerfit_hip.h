@@ -155,11 +155,14 @@ int pf_ctx_fused_available(pf_ctx *ctx);
  * from the batch size so that each launch has at least 4 blocks per CU).
  * The results do not depend on these values. */
 int pf_ctx_set_tuning(pf_ctx *ctx, int chunk_z, int chunk_lcc);
-/* Radix schedule of the fused z stage's FFTs: 0 (default) or 1 (an
- * alternative plan with a different element count per thread and radix
- * sequence, compiled for every fused size).  Picked by the autotuner; results
- * agree to FP32 rounding of the FFTs.  No counterpart in the reference. */
-#define PF_Z_SCHEDULES 2
+/* Schedule of the fused z stage, 0 .. PF_Z_SCHEDULES - 1: bit 0 selects the
+ * radix plan of its FFTs (the default, or an alternative with a different
+ * element count per thread and radix sequence), bit 1 the width of the
+ * (ky, kx) column tile per block (the default, or an alternative width); all
+ * four are compiled for every fused size, 0 being that size's measured best.
+ * Picked by the autotuner; the radix plan changes results by FP32 rounding of
+ * the FFTs, the tile width not at all.  No counterpart in the reference. */
+#define PF_Z_SCHEDULES 4
 int pf_ctx_set_schedule(pf_ctx *ctx, int z_schedule);
 int pf_ctx_profile(pf_ctx *ctx, int enable);
 int pf_ctx_kernel_time(pf_ctx *ctx, int which, double *ms, int64_t *launches);

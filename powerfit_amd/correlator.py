@@ -221,7 +221,8 @@ class HipCorrelator(BaseCorrelator):
                    "pf_ctx_set_tuning")
 
     def set_schedule(self, z_schedule=0):
-        """Radix schedule of the fused z-stage FFTs (0 default, 1 alternative)."""
+        """Schedule of the fused z stage (0 .. 3; bit 0 the alternative radix
+        plan, bit 1 the alternative column tile width; 0 is the default)."""
         _lib.check(self._libref.pf_ctx_set_schedule(self._ctx, int(z_schedule)),
                    "pf_ctx_set_schedule")
         self.z_schedule = int(z_schedule)

@@ -1025,8 +1025,9 @@ void launch_zcorr(const FusedArgs& a, int ch, int zr_read, const float2* twN, hi
 // Launch shape of one fused size.  K1: NTHR1 threads, element bounds EMAX (rows,
 // and the two-plane kernels) and EMAXC1 (one-plane column pass), mask values
 // kept in global memory and one launch per grid when STASHW.  K2: TQ columns per
-// block, element bounds EMAXZ / EMAXZ2 of the default and the alternative z
-// schedule (FusedArgs::zsched picks one).  K3: NTHR3 threads, element bounds
+// block (TQ2 the alternative width), element bounds EMAXZ / EMAXZ2 of the
+// default and the alternative radix plan (FusedArgs::zsched: bit 0 the radix
+// plan, bit 1 the tile width).  K3: NTHR3 threads, element bounds
 // EMAX3 (rows) / EMAXC3 (columns), options XPF3, ISDG3 and RECOMP3 of the
 // one-plane kernel.  Sizes that differ from the defaults derive from Shape and
 // redefine members.
@@ -1034,25 +1035,26 @@ template <int NTHR1_, int TQ_, int EZ_, int NTHR3_, int E_ = 16, int EZ2_ = EZ_>
 struct Shape {
     static constexpr int NTHR1 = NTHR1_, TQ = TQ_, EMAXZ = EZ_, NTHR3 = NTHR3_, EMAX = E_,
                          EMAXZ2 = EZ2_, EMAX3 = E_, EMAXC3 = E_, EMAXC1 = E_;
+    static constexpr int TQ2 = TQ_ == 32 ? 16 : 32;   // alternative z tile width
     static constexpr bool XPF3 = true, ISDG3 = false, RECOMP3 = false, STASHW = false;
     static constexpr bool PAIR3 = true;   // two-plane K3 where its plan fits
 };
 
-struct Shape80 : Shape<320, 32, 16, 320, 12, 24> {
+struct Shape80 : Shape<320, 16, 16, 320, 12, 24> {
     static constexpr bool RECOMP3 = true;
 };
 struct Shape128 : Shape<1024, 32, 16, 1024, 16, 8> {};
-struct Shape144 : Shape<864, 32, 12, 864, 16, 6> {
+struct Shape144 : Shape<864, 16, 12, 864, 16, 6> {
     static constexpr bool RECOMP3 = true;
 };
 // 160 = 2^5 * 5 has no 16-element plan; K3 sweeps its 80 columns once with
 // 20 elements per thread and keeps the variance in global memory
-struct Shape160 : Shape<704, 32, 10, 640, 20, 20> {
+struct Shape160 : Shape<704, 16, 10, 640, 20, 20> {
     static constexpr int EMAX3 = 10, EMAXC3 = 20;
     static constexpr bool XPF3 = false, ISDG3 = true, STASHW = true;
 };
 // 192: K3 runs its 96 columns in two sweeps of 48 (12 elements per thread)
-struct Shape192 : Shape<784, 32, 12, 768, 24, 24> {
+struct Shape192 : Shape<784, 16, 12, 768, 24, 24> {
     static constexpr int EMAX3 = 12, EMAXC3 = 12;
     static constexpr bool XPF3 = false, ISDG3 = true, STASHW = true;
 };
@@ -1085,7 +1087,12 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     }
     if (which < 0 || which == 1) {
         const int ch = a.chunk_z;
-        if (a.zsched == 1)
+        const bool alt_plan = a.zsched & 1, alt_tile = a.zsched & 2;
+        if (alt_tile && alt_plan)
+            launch_zcorr<N, SH::TQ2, SH::EMAXZ2>(a, ch, zr_read, tw.N, s);
+        else if (alt_tile)
+            launch_zcorr<N, SH::TQ2, SH::EMAXZ>(a, ch, zr_read, tw.N, s);
+        else if (alt_plan)
             launch_zcorr<N, TQ, SH::EMAXZ2>(a, ch, zr_read, tw.N, s);
         else
             launch_zcorr<N, TQ, SH::EMAXZ>(a, ch, zr_read, tw.N, s);

@@ -116,7 +116,7 @@ struct FusedArgs {
                                  // where |signed z| > zr (and rows where z^2+y^2 > zr^2)
     bool skip_planes;            // forward stage skips zero planes and the z stage reads
                                  // them as zero (both stages fused)
-    int zsched;                  // z-FFT schedule of the z stage: 0 default, 1 alternative
+    int zsched;                  // z-stage schedule: bit 0 alternative radix plan, bit 1 tile
     bool binary_mask;            // mask values all in {0, 1}: W^2 = W
     float2* work;                // [nrot][3][H] spectra
     const float2* F1;

@@ -360,16 +360,17 @@ SCHEDULE_CASES = [("f48", 48), ("cpu64", 64), ("f72", 72), ("f80", 80), ("lc96",
 
 @pytest.mark.parametrize("name,n", SCHEDULE_CASES)
 def test_alternative_z_schedule_vs_oracle(name, n):
-    """The alternative radix schedule of the fused z stage at every fused size:
-    oracle parity, and agreement with the default schedule to FP32 rounding."""
+    """Every z-stage schedule (radix plan x column tile) at every fused size:
+    oracle parity, agreement with the default to FP32 rounding, and the tile
+    width alone changes nothing."""
     w = NEW_SIZES[n] if n in NEW_SIZES else WORKLOADS[name]
     c = make_case(w)
     R = rotations_for_angle(w.angle)[:24]
     ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
                            laplace=c.laplace, nproc=NPROC)
-    say(f"scan {n}^3 fused, z schedules 0 and 1, 24 rotations")
+    say(f"scan {n}^3 fused, z schedules 0-3, 24 rotations")
     outs = []
-    for zs in (0, 1):
+    for zs in range(4):
         corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused",
                              batch=8)
         corr.set_schedule(zs)
@@ -379,8 +380,12 @@ def test_alternative_z_schedule_vs_oracle(name, n):
         corr.scan()
         err, dec = assert_scan_parity(corr.lcc, corr.rot, ref)
         say(f"  schedule {zs}: max err {err:.2e}, decisive voxels {dec}")
-        outs.append(corr.lcc.copy())
-    assert np.abs(outs[0] - outs[1]).max() < 1e-5
+        outs.append((corr.lcc.copy(), corr.rot.copy()))
+    for zs in (1, 3):
+        assert np.abs(outs[0][0] - outs[zs][0]).max() < 1e-5
+    for plan in (0, 1):   # schedules plan and plan + 2 differ in the tile width only
+        assert np.array_equal(outs[plan][0], outs[plan + 2][0])
+        assert np.array_equal(outs[plan][1], outs[plan + 2][1])
 
 
 LARGE_MIX = [(160, "mix_fwd"), (160, "mix_inv"), (192, "mix_fwd"), (192, "mix_inv")]
