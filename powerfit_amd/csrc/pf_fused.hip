@@ -1040,8 +1040,11 @@ struct Shape {
     static constexpr bool PAIR3 = true;   // two-plane K3 where its plan fits
 };
 
+struct Shape72 : Shape<384, 32, 12, 384, 16, 6> {
+    static constexpr bool STASHW = true;
+};
 struct Shape80 : Shape<320, 16, 16, 320, 12, 24> {
-    static constexpr bool RECOMP3 = true;
+    static constexpr bool RECOMP3 = true, STASHW = true;
 };
 struct Shape128 : Shape<1024, 32, 16, 1024, 16, 8> {};
 struct Shape144 : Shape<864, 16, 12, 864, 16, 6> {
@@ -1140,7 +1143,7 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
     switch (a.n) {
         case 48: return run_fused<48, Shape<192, 32, 12, 192, 16, 6>>(a, which, s);
         case 64: return run_fused<64, Shape<256, 32, 8, 256, 16, 4>>(a, which, s);
-        case 72: return run_fused<72, Shape<384, 32, 12, 384, 16, 6>>(a, which, s);
+        case 72: return run_fused<72, Shape72>(a, which, s);
         case 80: return run_fused<80, Shape80>(a, which, s);
         case 96: return run_fused<96, Shape<384, 32, 12, 384, 16, 24>>(a, which, s);
         case 128: return run_fused<128, Shape128>(a, which, s);
