@@ -1037,7 +1037,7 @@ struct Shape {
                          EMAXZ2 = EZ2_, EMAX3 = E_, EMAXC3 = E_, EMAXC1 = E_;
     static constexpr int TQ2 = TQ_ == 32 ? 16 : 32;   // alternative z tile width
     static constexpr bool XPF3 = true, ISDG3 = false, RECOMP3 = false, STASHW = false;
-    static constexpr bool PAIR3 = true;   // two-plane K3 where its plan fits
+    static constexpr bool PAIR1 = true, PAIR3 = true;   // two-plane K1 / K3 where they fit
 };
 
 struct Shape72 : Shape<384, 32, 12, 384, 16, 6> {
@@ -1046,7 +1046,11 @@ struct Shape72 : Shape<384, 32, 12, 384, 16, 6> {
 struct Shape80 : Shape<320, 16, 16, 320, 12, 24> {
     static constexpr bool RECOMP3 = true, STASHW = true;
 };
-struct Shape128 : Shape<1024, 32, 16, 1024, 16, 8> {};
+// 128: one-plane K1 in 512-thread blocks, one launch per grid (7.3 against
+// 8.4 us per rotation for the two-plane K1)
+struct Shape128 : Shape<512, 32, 16, 1024, 16, 8> {
+    static constexpr bool PAIR1 = false, STASHW = true;
+};
 struct Shape144 : Shape<864, 16, 12, 864, 16, 6> {
     static constexpr bool RECOMP3 = true;
 };
@@ -1075,7 +1079,7 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     const int zlo = skip ? -zr : 0;
     const int zr_read = skip ? zr : N;
     if (which < 0 || which == 0) {
-        if (a.binary_mask && PairPlan<N, NTHR1, EMAX>::ok)
+        if (a.binary_mask && SH::PAIR1 && PairPlan<N, NTHR1, EMAX>::ok)
             launch_fwd_pair<N, NTHR1, EMAX>(dim3(nplanes, a.nrot), s, a.tm, a.rots, a.rmax, a.zr,
                                             zlo, a.work, tw.M, tw.N, a.occ);
         else if (a.binary_mask)
