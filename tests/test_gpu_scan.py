@@ -247,6 +247,29 @@ def test_graph_replays_match_eager(mode):
     assert_scan_parity(*eager, ref)
 
 
+@pytest.mark.parametrize("n", [128, 192])
+def test_graph_replays_match_eager_multi_launch_stages(n):
+    """Graph replays where K1 runs one launch per grid (128, 192) and K3 keeps
+    the variance in the work buffer (192): every replay equals the eager scan."""
+    w = WORKLOADS["search128"] if n == 128 else NEW_SIZES[192]
+    c = make_case(w)
+    R = rotations_for_angle(w.angle)[:32]
+    say(f"graph capture {n}^3 fused, 32 rotations")
+    corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused", batch=16)
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    corr.scan()
+    eager = (corr.lcc.copy(), corr.rot.copy())
+    for rep in range(3):
+        corr.reset()
+        corr.enqueue_scan_graph(0, len(R))
+        lcc, rot = corr.result()
+        assert np.array_equal(lcc, eager[0]) and np.array_equal(rot, eager[1]), rep
+        corr.scan()
+        assert np.array_equal(corr.lcc, eager[0])
+
+
 def test_serving_results_equal_standalone():
     """Fits admitted together and interleaved on the device give exactly the
     result each gives alone."""
