@@ -1146,7 +1146,7 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         return kErrArg;
     }
     switch (a.n) {
-        case 48: return run_fused<48, Shape<192, 32, 12, 192, 16, 6>>(a, which, s);
+        case 48: return run_fused<48, Shape<192, 32, 12, 96, 16, 6>>(a, which, s);
         case 64: return run_fused<64, Shape<256, 32, 8, 256, 16, 4>>(a, which, s);
         case 72: return run_fused<72, Shape72>(a, which, s);
         case 80: return run_fused<80, Shape80>(a, which, s);
