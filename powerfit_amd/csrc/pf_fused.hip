@@ -1150,7 +1150,7 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         case 64: return run_fused<64, Shape<256, 32, 8, 256, 16, 4>>(a, which, s);
         case 72: return run_fused<72, Shape72>(a, which, s);
         case 80: return run_fused<80, Shape80>(a, which, s);
-        case 96: return run_fused<96, Shape<384, 32, 12, 384, 16, 24>>(a, which, s);
+        case 96: return run_fused<96, Shape<768, 32, 12, 384, 16, 24>>(a, which, s);
         case 128: return run_fused<128, Shape128>(a, which, s);
         case 144: return run_fused<144, Shape144>(a, which, s);
         case 160: return run_fused<160, Shape160>(a, which, s);
