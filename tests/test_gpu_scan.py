@@ -377,6 +377,29 @@ def test_fused_serving_sizes_vs_oracle(n):
     say(f"  max err {err:.2e}, decisive voxels {dec}")
 
 
+BINARY_SIZES = {
+    72: Workload("b72", 72, 1.8, 2000, 7.0, 20.0, seed=7650),
+    96: Workload("b96", 96, 1.5, 3000, 6.0, 20.0, seed=7950),
+    192: Workload("b192", 192, 1.0, 10000, 5.0, 20.0, seed=8050),
+}
+
+
+@pytest.mark.parametrize("n", sorted(BINARY_SIZES))
+def test_fused_binary_mask_sizes_vs_oracle(n):
+    """Sizes whose other GPU cases use weighted masks: the 0/1-mask form of
+    their first kernel (two-plane at 72, one launch per grid at 192)."""
+    w = BINARY_SIZES[n]
+    c = make_case(w)
+    assert set(np.unique(c.mask)) <= {0.0, 1.0}
+    R = rotations_for_angle(w.angle)[:24]
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
+                           laplace=c.laplace, nproc=NPROC)
+    say(f"scan {n}^3 fused, 0/1 mask, 24 rotations")
+    best, rot = run_hip(c, R, "fused", 8)
+    err, dec = assert_scan_parity(best, rot, ref)
+    say(f"  max err {err:.2e}, decisive voxels {dec}")
+
+
 SCHEDULE_CASES = [("f48", 48), ("cpu64", 64), ("f72", 72), ("f80", 80), ("lc96", 96),
                   ("search128", 128), ("f144", 144), ("f160", 160), ("f192", 192)]
 
