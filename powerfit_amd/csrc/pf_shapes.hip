@@ -264,7 +264,7 @@ struct DevBuf {
 int run_shape(bool blur, int device, const double* points, const double* wr, int64_t n,
               double sigma, double* out, int nz, int ny, int nx, int wraparound) {
     if (!out || nz <= 0 || ny <= 0 || nx <= 0) {
-        set_error("shape generation: bad grid");
+        set_error("shape generation: output grid is null or has a dimension below 1");
         return kErrArg;
     }
     if (n < 0 || (n > 0 && (!points || !wr))) {

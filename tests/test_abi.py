@@ -39,3 +39,29 @@ def test_ctx_bytes_model():
     plain = lib.pf_ctx_bytes(128, 128, 128, 64, _lib.MODES["plain"])
     assert 0 < plain < fused < unfused
     assert lib.pf_ctx_bytes(0, 1, 1, 1, 0) == -1
+
+
+def test_more_argument_errors_need_no_device():
+    """Every entry point rejects bad arguments before touching a device."""
+    lib = _lib.load()
+    pts, w = np.zeros((3, 2)), np.ones(2)
+    grid = np.zeros((4, 4, 4))
+    rc = lib.pf_blur_points(0, _lib.ptr(pts, _lib.DP), _lib.ptr(w, _lib.DP), 2, 1.0, None,
+                            4, 4, 4, 0)
+    assert rc == -1 and b"output grid" in lib.pf_last_error()
+    rc = lib.pf_dilate_points(0, _lib.ptr(pts, _lib.DP), _lib.ptr(w, _lib.DP), -1,
+                              _lib.ptr(grid, _lib.DP), 4, 4, 4, 0)
+    assert rc == -1 and b"atom arrays" in lib.pf_last_error()
+    corr = np.zeros((4, 4, 4), np.float32)
+    count = ctypes.c_int64(0)
+    assert lib.pf_peak_positions(0, corr.ctypes.data, 7, 4, 4, 4, corr.ctypes.data, 1, None, 0,
+                                 ctypes.byref(count)) == -1
+    assert b"dtype" in lib.pf_last_error()
+    assert lib.pf_peak_positions(0, corr.ctypes.data, 0, 0, 4, 4, corr.ctypes.data, 1, None, 0,
+                                 ctypes.byref(count)) == -1
+    assert lib.pf_peak_positions(0, corr.ctypes.data, 0, 4, 4, 4, corr.ctypes.data, 1, None, 0,
+                                 None) == -1
+    for rc in (lib.pf_ctx_set_schedule(None, 1), lib.pf_ctx_set_tuning(None, 1, 1),
+               lib.pf_set_rotations(None, None, 3), lib.pf_scan_graph(None, 0, 1, 1, 0),
+               lib.pf_get_result(None, None, None), lib.pf_rfftn_dev(None, None, 0, 4, 4, 1, None)):
+        assert rc == -1
