@@ -112,7 +112,8 @@ class HipCorrelator(BaseCorrelator):
             kernels; cubic grids of a compiled size), or "auto" (fused where
             available, else batched).  "mix_fwd"/"mix_z"/"mix_inv" run one
             fused stage with the rest unfused (for testing).
-    batch : rotations per launch.
+    batch : rotations per launch (256 measured best at every fused size; a
+        scan of fewer rotations uses only as many).
     binary_shortcut : use FFT(W^2) = FFT(W) when the mask is 0/1.
     precision : "fp32" (default) or "split": the unfused y and z FFT passes
         do their first radix-16 stage as a DFT-as-GEMM on the matrix cores with
@@ -120,7 +121,7 @@ class HipCorrelator(BaseCorrelator):
         at most 320; the fused pipeline is not used).
     """
 
-    def __init__(self, target, laplace=False, device=0, mode="auto", batch=64,
+    def __init__(self, target, laplace=False, device=0, mode="auto", batch=256,
                  binary_shortcut=True, precision="fp32"):
         super().__init__(target, laplace=laplace)
         if mode not in _lib.MODES:

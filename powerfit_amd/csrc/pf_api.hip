@@ -463,6 +463,9 @@ int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     }
     if (mode == PF_MODE_PLAIN) batch = 1;
     if (batch < 1 || batch > 4096) return arg_error("batch must be in [1, 4096]");
+    // a shorter range needs no more work buffer than its own length (results
+    // do not depend on the batch size)
+    if (count > 0 && batch > count) batch = (int)count;
     // stage A: rotate + forward xy; B: z correlate; C: inverse xy + LCC
     const bool fa = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_FWD;
     const bool fb = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_Z;
@@ -598,6 +601,7 @@ int pf_ctx_set_schedule(pf_ctx* c, int z_schedule) {
 
 int pf_scan_graph(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
     if (!c) return arg_error("ctx is null");
+    if (count > 0 && batch > count) batch = (int)count;   // as pf_scan does
     if (c->profile) {
         set_error("pf_scan_graph: disable kernel timing first");
         return kErrState;

@@ -36,7 +36,7 @@ class PowerFitter:
         self._directory = abspath("./")
         self._laplace = laplace
         self.mode = "auto"
-        self.batch = 64
+        self.batch = 256
 
     @property
     def directory(self):
