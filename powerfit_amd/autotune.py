@@ -15,7 +15,7 @@ import time
 
 import numpy as np
 
-DEFAULT_BATCHES = (32, 64, 128, 256)
+DEFAULT_BATCHES = (32, 64, 128, 256, 512)
 DEFAULT_CHUNKS = ((0, 0), (4, 4), (8, 8), (16, 16))
 
 
