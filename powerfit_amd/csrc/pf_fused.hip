@@ -734,20 +734,48 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane2(const float2* __restrict__ 
 // consecutive floats of the slot per instruction (lane l -> float l of the
 // chunk), nothing passes through VGPRs.  Padding floats of a row receive a
 // copy of the row's first float.  The loads complete under vmcnt.
+//
+// The lane -> (row, float) pattern repeats every GR rows (GR * RF floats are a
+// whole number CG of 64-float chunks), so each lane computes its CG byte
+// offsets within such a group once per call; the copies go through a buffer
+// resource over the source plane, with the group's offset as the scalar
+// offset, so issuing them takes no vector arithmetic (and a copy outside the
+// plane's bytes would read zeros instead of faulting).
+constexpr int dma_gcd(int a, int b) { return b == 0 ? a : dma_gcd(b, a % b); }
+
 template <int N, int NTHR>
 __device__ __forceinline__ void dma_plane(float2* slot, const float2* __restrict__ src) {
     using G = Geo<N>;
     constexpr int RF = 2 * G::S;     // floats per LDS row
     constexpr int GF = 2 * G::NXH;   // floats per global row
-    constexpr int CHUNKS = N * RF / 64;
-    static_assert((N * RF) % 64 == 0, "whole 64-float chunks");
-    const float* s = reinterpret_cast<const float*>(src);
+    constexpr int GR = 64 / dma_gcd(RF, 64);   // rows per group
+    constexpr int CG = GR * RF / 64;            // chunks per group
+    constexpr int NG = N / GR;                  // groups per plane
+    constexpr int NW = NTHR / 64;               // waves
+    static_assert(N % GR == 0 && NTHR % 64 == 0, "whole groups of rows, whole waves");
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float2*>(src), 0, (int)(sizeof(float) * N * GF), 0x00020000);
     float* d = reinterpret_cast<float*>(slot);
-    const int lane = threadIdx.x & 63;
-    for (int c = threadIdx.x >> 6; c < CHUNKS; c += NTHR / 64) {
-        const int f = c * 64 + lane;
+    // through an empty asm, so the offsets are computed per call instead of
+    // being hoisted out of the caller's loop (where they would hold CG
+    // registers across it)
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    int off[CG];
+#pragma unroll
+    for (int j = 0; j < CG; ++j) {
+        const int f = j * 64 + lane;
         const int row = f / RF, fc = f - row * RF;
-        __builtin_amdgcn_global_load_lds(s + row * GF + (fc < GF ? fc : 0), d + c * 64, 4, 0, 0);
+        off[j] = 4 * (row * GF + (fc < GF ? fc : 0));
+    }
+    // the wave index, as a wave-uniform (scalar) value
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int g = wave; g < NG; g += NW) {
+#pragma unroll
+        for (int j = 0; j < CG; ++j)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rsrc, (__attribute__((address_space(3))) void*)(d + g * (GR * RF) + j * 64), 4,
+                off[j], 4 * g * (GR * GF), 0, 0);
     }
 }
 
