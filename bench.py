@@ -30,10 +30,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def alg_bytes_per_launch(kernel, V, H, B, binary=False):
-    """Algorithmic bytes one launch of a kernel class must move (f32 = 4 B,
-    complex64 = 8 B), for B rotations per launch.  With a 0/1 mask the fused
-    path transforms W once (FFT(W^2) = FFT(W)): 2 xy-spectra instead of 3."""
+def alg_bytes(kernel, V, H, binary=False):
+    """(bytes per rotation, bytes per launch) a kernel class must move
+    (f32 = 4 B, complex64 = 8 B).  With a 0/1 mask the fused path transforms W
+    once (FFT(W^2) = FFT(W)): 2 xy-spectra instead of 3.  The per-launch part
+    is what a launch reads or writes once whatever its rotation count (target
+    spectra, lcc mask, the key read and write)."""
     nsp = 2 if binary else 3
     per_rot = {
         "rotate": 8 * V + 12 * V,          # read template+mask, write P, W, W^2
@@ -50,13 +52,14 @@ def alg_bytes_per_launch(kernel, V, H, B, binary=False):
     }[kernel]
     per_launch = {"conj_mul": 16 * H, "lcc": V + 16 * V, "zcorr": 16 * H,
                   "inv_plane": V + 16 * V}.get(kernel, 0)  # F1,F2 / mask, keys r+w
-    return per_rot * B + per_launch
+    return per_rot, per_launch
 
 
-def measured_traffic(workload, mode, kernel, B, binary):
+def measured_traffic(workload, mode, kernel, rotations_per_launch, binary):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (profiles/<workload>_fused_traffic.json), scaled to B rotations;
-    None when no summary covers this configuration."""
+    summary (profiles/<workload>_fused_traffic.json, bytes per rotation),
+    at the average rotations per launch; None when no summary covers this
+    configuration."""
     path = os.path.join(ROOT, "profiles", f"{workload}_fused_traffic.json")
     if mode not in ("fused", "auto") or not os.path.exists(path):
         return None
@@ -65,7 +68,7 @@ def measured_traffic(workload, mode, kernel, B, binary):
     k = prof.get("kernels", {}).get(kernel)
     if k is None or prof.get("binary_mask") != binary:
         return None
-    return k["bytes_per_rotation"] * B
+    return k["bytes_per_rotation"] * rotations_per_launch
 
 
 def path_bytes_per_rotation(V, H):
@@ -89,9 +92,34 @@ def run_scan(corr, K, B, steps, first0=0):
 # ---- the CPU leg: the only place bench.py runs the oracle (FP64 numpy
 # restatement of the reference's CPU path) -- timed as the CPU baseline, and
 # as the checker of GPU results on samples
+def cpu_workers():
+    """Host cores for the CPU baseline: the cores this process may run on,
+    at most 16 (a one-GPU share of the host; os.cpu_count() reports the whole
+    machine)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(case, R, nproc, per_worker):
+    """The oracle (FP64 numpy, the reference's CPU fallback path) with nproc
+    forked workers and the reference's contiguous chunking over the first
+    nproc * per_worker rotations; returns (rotations, seconds)."""
     from oracle import correlator as oc
-    n = nproc * per_worker
+    n = min(nproc * per_worker, R.shape[0])
     done, secs = oc.timed_parallel_scan(case.target.astype(np.float64), case.template, case.mask,
                                         R[:n], laplace=case.laplace, nproc=nproc)
     return done, secs
@@ -148,9 +176,8 @@ def precision_report(args, HipCorrelator, case, R, w):
         rows[key] = {"rotations_per_s": n / best, "path": f"{mode}, batch {args.precision_batch}"}
         del corr
     from oracle import correlator as oc
-    nproc = min(16, os.cpu_count() or 1)
     ref = oc.parallel_scan(case.target.astype(np.float64), case.template, case.mask, Rs,
-                           laplace=case.laplace, nproc=nproc)
+                           laplace=case.laplace, nproc=cpu_workers())
     for key in rows:
         rows[key].update(cpu_baseline_check(case, Rs, *results[key], ref=ref))
     nz, ny, nx = case.target.shape
@@ -167,6 +194,11 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=0,
                     help="rotations per launch (0 = autotune batch and chunk sizes)")
+    ap.add_argument("--chunks", default=None,
+                    help="rotations per block of the fused z and LCC kernels, 'cz,cl' "
+                         "(with --batch; default automatic)")
+    ap.add_argument("--z-schedule", type=int, default=0,
+                    help="schedule of the fused z stage (with --batch)")
     ap.add_argument("--mode", default="fused", choices=["fused", "batched", "plain", "auto"])
     ap.add_argument("--workload", default="search128")
     ap.add_argument("--plain-rotations", type=int, default=256,
@@ -176,8 +208,9 @@ def main(argv=None):
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 256 MiB device-to-device copy first (calibrates HBM byte "
                          "counters under a profiler)")
-    ap.add_argument("--cpu-per-worker", type=int, default=2,
-                    help="rotations per CPU worker in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-per-worker", type=int, default=64,
+                    help="rotations per CPU worker in the CPU baseline sample: the first "
+                         "64 P rotations with P workers (0 = skip)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "split"],
                     help="FFT arithmetic of the measured path: fp32, or split (first radix-16 "
                          "stage of the unfused y/z passes as an FP16 hi+lo GEMM on MFMA)")
@@ -225,6 +258,10 @@ def main(argv=None):
         B = 1
     elif args.batch > 0:
         B = args.batch
+        if args.chunks:
+            cz, cl = (int(v) for v in args.chunks.split(","))
+            corr.set_tuning(cz, cl)
+        corr.set_schedule(args.z_schedule)
     else:
         from powerfit_amd.autotune import autotune
         tuning = autotune(corr, use_cache=False)
@@ -252,12 +289,15 @@ def main(argv=None):
     rot_done = steps * B
     value = rot_done / elapsed
 
-    # dominant kernel roofline
+    # dominant kernel roofline: algorithmic bytes of every launch of the
+    # kernel in the timed region (per rotation x rotations run, plus the
+    # per-launch part x launches) over the kernel's summed event time
     dom = max(ktimes, key=lambda k: ktimes[k][0])
     dom_ms, dom_n = ktimes[dom]
-    avg_s = dom_ms / 1e3 / dom_n
     binary = bool(np.all((case.mask == 0) | (case.mask == 1)))
-    achieved = alg_bytes_per_launch(dom, V, H, B, binary) / avg_s / 1e9
+    per_rot, per_launch = alg_bytes(dom, V, H, binary)
+    dom_bytes = per_rot * rot_done + per_launch * dom_n
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     path_gbs = value * path_bytes_per_rotation(V, H) / 1e9
 
     # plain one-rotation-per-launch path on the same inputs
@@ -304,12 +344,14 @@ def main(argv=None):
 
     cpu = None
     if args.cpu_per_worker > 0:
-        nproc = min(16, os.cpu_count() or 1)
+        nproc = cpu_workers()
         done, secs = cpu_baseline(case, R, nproc, args.cpu_per_worker)
         cpu = {"value": done / secs, "unit": "rotations/s", "cores": nproc, "kind": "port",
-               "sample": f"first {done} rotations of the {K}-rotation set, {nproc} forked "
-                         f"FP64 numpy workers with the reference's contiguous chunking, "
-                         f"setup excluded ({secs:.1f} s wall)"}
+               "cpu_model": cpu_model(),
+               "sample": f"prefix-timed: first {done} rotations ({args.cpu_per_worker} per "
+                         f"worker) of the {K}-rotation set, {nproc} forked FP64 numpy workers "
+                         f"with the reference's contiguous chunking, setup excluded "
+                         f"({secs:.1f} s wall)"}
 
     out = {
         "metric": "rotations/sec (LCC search, 128^3 grid) + HBM roofline %, chosen vs plain FP32",
@@ -335,8 +377,12 @@ def main(argv=None):
                    "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "alg_bytes_per_launch": alg_bytes_per_launch(dom, V, H, B, binary),
-                     "traffic": measured_traffic(args.workload, args.mode, dom, B, binary)},
+                     "launches": dom_n, "rotations": rot_done,
+                     "kernel_ms_per_launch": dom_ms / dom_n,
+                     "alg_bytes_per_rotation": per_rot, "alg_bytes_per_launch_fixed": per_launch,
+                     "alg_bytes_per_launch": dom_bytes / dom_n,
+                     "traffic": measured_traffic(args.workload, args.mode, dom, rot_done / dom_n,
+                                                 binary)},
         "path_roofline": {"bytes_per_rotation": path_bytes_per_rotation(V, H),
                           "achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": path_gbs / HBM_PEAK_GBS},

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define PF_ABI_VERSION 1
+#define PF_ABI_VERSION 2
 
 #define PF_OK 0
 #define PF_ERR_ARG (-1)
@@ -120,9 +120,12 @@ int pf_reset_result(pf_ctx *ctx);
 int pf_scan(pf_ctx *ctx, int64_t first, int64_t count, int batch, int mode);
 
 /* pf_scan recorded once into a HIP graph (per ctx, for the last
- * (first, count, batch, mode) used) and replayed: same launches, one graph
- * launch.  Needs the work buffers sized beforehand (done here, outside the
- * capture) and kernel timing off.  Enqueues only. */
+ * (first, count, batch, mode) and tuning used) and replayed: same launches,
+ * one graph launch.  Needs the work buffers sized beforehand (done here,
+ * outside the capture) and kernel timing off.  pf_set_rotations,
+ * pf_set_template_mask and a work-buffer reallocation discard the captured
+ * graph (its launches name those buffers and values); the next call captures
+ * again.  Enqueues only. */
 int pf_scan_graph(pf_ctx *ctx, int64_t first, int64_t count, int batch, int mode);
 
 /* Copy out best LCC (float32) and rotation index (int32), shape (nz, ny, nx)
@@ -132,9 +135,22 @@ int pf_sync(pf_ctx *ctx);
 /* Non-blocking: 1 when all work enqueued on the ctx stream has finished,
  * 0 when some is still running (hipStreamQuery). */
 int pf_ctx_query(pf_ctx *ctx);
-/* Device bytes a ctx for an (nz, ny, nx) grid allocates when scanning with
- * `batch` rotations per launch in `mode` (for admission control). */
-int64_t pf_ctx_bytes(int nz, int ny, int nx, int batch, int mode);
+/* Peak device bytes of a ctx for an (nz, ny, nx) grid created with `flags`,
+ * holding `nrot` rotations and scanning with `batch` rotations per launch in
+ * `mode` (resident inputs and keys, plus the larger of the scan's work
+ * buffers and pf_set_target's transient buffers), for admission control.
+ * No counterpart in the reference (its GPU path allocates per correlator,
+ * powerfitter.py:432-443, without a budget). */
+int64_t pf_ctx_bytes(int nz, int ny, int nx, int batch, int mode, uint32_t flags, int64_t nrot);
+
+/* Progress marks on the ctx stream, for callers that keep several batches in
+ * flight (the serving loop): pf_ctx_mark records a point after everything
+ * enqueued so far and returns its ticket (increasing from 1); pf_ctx_reached
+ * returns 1 once the work before that mark has finished, else 0 (it does not
+ * block).  No counterpart in the reference, whose GPU loop blocks on
+ * queue.finish() per rotation (powerfitter.py:513-538). */
+int pf_ctx_mark(pf_ctx *ctx, uint64_t *ticket);
+int pf_ctx_reached(pf_ctx *ctx, uint64_t ticket);
 
 /* Kernel timing of subsequent pf_scan calls with HIP events on the ctx
  * stream (0 = off).  pf_ctx_kernel_time returns, for kernel class `which`

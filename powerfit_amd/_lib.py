@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpowerfit_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "powerfit_hip.h")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MODES = {"plain": 0, "batched": 1, "fused": 2, "auto": 3,
          "mix_fwd": 4, "mix_z": 5, "mix_inv": 6}
 MODE_PLAIN = MODES["plain"]
@@ -53,7 +53,9 @@ _SIGNATURES = {
     "pf_ctx_fused_available": ([_vp], _c_int),
     "pf_ctx_set_tuning": ([_vp, _c_int, _c_int], _c_int),
     "pf_ctx_query": ([_vp], _c_int),
-    "pf_ctx_bytes": ([_c_int, _c_int, _c_int, _c_int, _c_int], _c_i64),
+    "pf_ctx_bytes": ([_c_int, _c_int, _c_int, _c_int, _c_int, _c_u32, _c_i64], _c_i64),
+    "pf_ctx_mark": ([_vp, ctypes.POINTER(ctypes.c_uint64)], _c_int),
+    "pf_ctx_reached": ([_vp, ctypes.c_uint64], _c_int),
     "pf_ctx_kernel_time": ([_vp, _c_int, _dp, ctypes.POINTER(_c_i64)], _c_int),
     "pf_ctx_set_schedule": ([_vp, _c_int], _c_int),
     "pf_blur_points": ([_c_int, _dp, _dp, _c_i64, _c_dbl, _dp, _c_int, _c_int, _c_int, _c_int],

@@ -244,6 +244,20 @@ class HipCorrelator(BaseCorrelator):
             _lib.check(rc, "pf_ctx_query")
         return rc == 1
 
+    def mark(self):
+        """Progress ticket for the work enqueued so far (see ``reached``)."""
+        t = ctypes.c_uint64(0)
+        _lib.check(self._libref.pf_ctx_mark(self._ctx, ctypes.byref(t)), "pf_ctx_mark")
+        return t.value
+
+    def reached(self, ticket):
+        """True once the work enqueued before ``mark()`` returned `ticket` has
+        finished (does not block)."""
+        rc = self._libref.pf_ctx_reached(self._ctx, int(ticket))
+        if rc < 0:
+            _lib.check(rc, "pf_ctx_reached")
+        return rc == 1
+
     def result(self):
         """Copy out (lcc float32, rot int32) of the work enqueued so far (syncs)."""
         self._lcc = np.zeros(self._target.shape, dtype=np.float32)

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <deque>
 #include <vector>
 
 #include "../../include/powerfit_hip.h"
@@ -57,6 +58,14 @@ struct pf_ctx {
     std::vector<hipEvent_t> pool;
     double acc_ms[PF_K_COUNT] = {};
     int64_t acc_n[PF_K_COUNT] = {};
+    // progress marks (pf_ctx_mark / pf_ctx_reached): events in ticket order
+    struct Mark {
+        uint64_t ticket;
+        hipEvent_t ev;
+    };
+    std::deque<Mark> marks;
+    std::vector<hipEvent_t> mark_pool;
+    uint64_t next_ticket = 1, reached = 0;
 };
 
 namespace {
@@ -146,7 +155,18 @@ int drain_profile(pf_ctx* c) {
     return kOk;
 }
 
+// Drop the captured scan: its kernel arguments name buffers and values that
+// the caller is about to replace (rotations, template and mask, work buffers).
+void drop_graph(pf_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->g_first = c->g_count = -1;
+}
+
 int ensure_capacity(pf_ctx* c, int batch, bool need_real) {
+    if (batch > c->cap_cplx || (need_real && batch > c->cap_real)) drop_graph(c);
     if (batch > c->cap_cplx) {
         PF_HIP_CHECK(hipStreamSynchronize(c->stream));
         dev_free(c->d_cplx);
@@ -305,8 +325,9 @@ int pf_ctx_destroy(pf_ctx* c) {
             (void)hipEventDestroy(r.b);
         }
         for (auto e : c->pool) (void)hipEventDestroy(e);
-        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-        if (c->graph) (void)hipGraphDestroy(c->graph);
+        for (auto& m : c->marks) (void)hipEventDestroy(m.ev);
+        for (auto e : c->mark_pool) (void)hipEventDestroy(e);
+        drop_graph(c);
         dev_free(c->d_template);
         dev_free(c->d_mask);
         dev_free(c->d_F1);
@@ -372,6 +393,8 @@ int pf_set_template_mask(pf_ctx* c, const double* tmpl, const double* mask, doub
     if (!radius_ok(rmax, c->nz, c->ny, c->nx))
         return arg_error("rmax must satisfy 0 <= rmax <= min(shape) - 2");
     PF_DEVICE_GUARD(c);
+    PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    drop_graph(c);   // the captured launches carry norm, rmax and the support radius
     double* dt = nullptr;
     int s = dev_alloc(&dt, c->V);
     if (!s && hipMemcpyAsync(dt, tmpl, sizeof(double) * c->V, hipMemcpyHostToDevice,
@@ -426,6 +449,7 @@ int pf_set_rotations(pf_ctx* c, const double* rot, int64_t count) {
     if (count < 0 || count > 0x7fffffffLL) return arg_error("invalid rotation count");
     PF_DEVICE_GUARD(c);
     PF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    drop_graph(c);   // the captured launches point into the old rotation array
     dev_free(c->d_rot);
     c->nrot = 0;
     if (count == 0) return kOk;
@@ -570,17 +594,69 @@ int pf_ctx_query(pf_ctx* c) {
     return kErrHip;
 }
 
-int64_t pf_ctx_bytes(int nz, int ny, int nx, int batch, int mode) {
-    if (nz < 1 || ny < 1 || nx < 1 || batch < 1) return arg_error("invalid shape or batch");
+int64_t pf_ctx_bytes(int nz, int ny, int nx, int batch, int mode, uint32_t flags, int64_t nrot) {
+    if (nz < 1 || ny < 1 || nx < 1 || batch < 1 || nrot < 0)
+        return arg_error("invalid shape, batch or rotation count");
     const int64_t V = (int64_t)nz * ny * nx, H = (int64_t)nz * ny * (nx / 2 + 1);
     if (mode == PF_MODE_PLAIN) batch = 1;
+    if (nrot > 0 && batch > nrot) batch = (int)nrot;   // pf_scan never uses more
+    // the fused work layout holds only the complex spectra; every other mode,
+    // and the split-precision FFT, also needs the real grids
     const bool fused = (mode == PF_MODE_FUSED || mode == PF_MODE_AUTO) &&
-                       fused_supported(nz, ny, nx);
-    // template, mask (f32), packed pair, keys, lcc mask, two target spectra
-    int64_t bytes = V * (4 + 4 + 8 + 8 + 1) + 2 * H * 8;
-    bytes += (int64_t)3 * batch * H * 8;             // complex work
-    if (!fused) bytes += (int64_t)3 * batch * V * 4;  // real work of the unfused stages
-    return bytes;
+                       !(flags & PF_FLAG_SPLIT_DFT) && fused_supported(nz, ny, nx);
+    // template, mask (f32), packed pair, keys, lcc mask, two target spectra,
+    // brick occupancy, rotations
+    int64_t bytes = V * (4 + 4 + 8 + 8 + 1) + 2 * H * 8 + nrot * 9 * 8;
+    if (fused_supported(nz, ny, nx)) bytes += 4 * (((int64_t)(nx / 8) * (nx / 8) * (nx / 8) + 31) / 32);
+    // work buffers (allocated by the first scan) or, before that, the
+    // transient buffers of pf_set_target (f64 upload + two f32 grids) and
+    // pf_set_template_mask (f64 upload), whichever is larger
+    int64_t work = (int64_t)3 * batch * H * 8;
+    if (!fused) work += (int64_t)3 * batch * V * 4;
+    return bytes + std::max(work, 16 * V);
+}
+
+int pf_ctx_mark(pf_ctx* c, uint64_t* ticket) {
+    if (!c || !ticket) return arg_error("null pointer");
+    PF_DEVICE_GUARD(c);
+    hipEvent_t e = nullptr;
+    if (!c->mark_pool.empty()) {
+        e = c->mark_pool.back();
+        c->mark_pool.pop_back();
+    } else {
+        PF_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    const hipError_t r = hipEventRecord(e, c->stream);
+    if (r != hipSuccess) {
+        c->mark_pool.push_back(e);
+        set_error("hipEventRecord: %s", hipGetErrorString(r));
+        return kErrHip;
+    }
+    *ticket = c->next_ticket++;
+    c->marks.push_back({*ticket, e});
+    return kOk;
+}
+
+int pf_ctx_reached(pf_ctx* c, uint64_t ticket) {
+    if (!c) return arg_error("ctx is null");
+    if (ticket >= c->next_ticket) return arg_error("unknown ticket");
+    PF_DEVICE_GUARD(c);
+    // marks complete in stream order: retire them from the front
+    while (!c->marks.empty()) {
+        const hipError_t e = hipEventQuery(c->marks.front().ev);
+        if (e == hipErrorNotReady) {
+            (void)hipGetLastError();
+            break;
+        }
+        if (e != hipSuccess) {
+            set_error("hipEventQuery: %s", hipGetErrorString(e));
+            return kErrHip;
+        }
+        c->reached = c->marks.front().ticket;
+        c->mark_pool.push_back(c->marks.front().ev);
+        c->marks.pop_front();
+    }
+    return ticket <= c->reached ? 1 : 0;
 }
 
 int pf_ctx_set_tuning(pf_ctx* c, int chunk_z, int chunk_lcc) {

@@ -5,10 +5,12 @@ target spectra resident in HBM for the whole fit.  Requests are admitted as
 slots and memory free up — a fit is admitted when the bytes its context will
 allocate (``pf_ctx_bytes``) fit in the remaining HBM budget and a slot is
 free — so fits of mixed grid sizes share the device.  The loop interleaves
-the active fits batch by batch (each keeps at most ``depth`` batches queued on
-its stream), so work from several fits overlaps on the device and a request
-arriving mid-way starts as soon as it is admitted instead of waiting for the
-queue to drain.  A fit's result is read back when its stream is idle; its
+the active fits batch by batch: each keeps up to ``depth`` batches in flight
+on its stream, and a progress mark after every batch (``pf_ctx_mark``) lets
+the loop queue the next batch as soon as one finishes, so a fit's stream
+never runs dry while it has rotations left.  Work from several fits overlaps
+on the device, and a request arriving mid-way starts as soon as it is
+admitted instead of waiting for the queue to drain.  A fit's result is read back when its stream is idle; its
 context is then released and the memory returned to the budget — or, when
 the request names its map (``target_key``), kept resident with that map's
 spectra: a later request on the same map takes the idle context and only
@@ -72,24 +74,39 @@ class _Active:
     corr: HipCorrelator
     bytes: int
     admitted: float
+    batch: int
     next_rot: int = 0
-    queued: list = field(default_factory=list)
+    queued: list = field(default_factory=list)   # progress tickets of batches in flight
 
 
-def ctx_bytes(shape, batch, mode="auto"):
-    """Device bytes a context for `shape` allocates (admission control)."""
+def ctx_bytes(shape, batch, mode="auto", flags=0, nrot=0):
+    """Peak device bytes of a context for `shape` (admission control)."""
     nz, ny, nx = shape
-    b = _lib.load().pf_ctx_bytes(nz, ny, nx, int(batch), _lib.MODES[mode])
+    b = _lib.load().pf_ctx_bytes(nz, ny, nx, int(batch), _lib.MODES[mode], int(flags), int(nrot))
     if b < 0:
         _lib.check(int(b), "pf_ctx_bytes")
     return int(b)
 
 
+def fit_batch(shape, nrot, budget, max_active, mode="auto", cap=256):
+    """Rotations per launch for one fit: the largest power of two up to `cap`
+    (256: the measured best or within 1.5 % of it at every fused size) whose
+    context fits an equal share (budget / max_active) of the HBM budget, and
+    no more than the fit's rotations."""
+    share = budget // max(1, max_active)
+    b = cap
+    while b > 1 and ctx_bytes(shape, b, mode, 0, nrot) > share:
+        b //= 2
+    return max(1, min(b, int(nrot)))
+
+
 class FitServer:
     """Admit, interleave and complete LCC searches on one device."""
 
-    def __init__(self, device=0, hbm_budget=64 << 30, max_active=8, batch=64, depth=2,
+    def __init__(self, device=0, hbm_budget=64 << 30, max_active=8, batch=0, depth=2,
                  mode="auto", keep_targets=True):
+        """batch: rotations per launch for every fit, or 0 to pick per fit
+        (``fit_batch``: up to 256 within an equal share of the budget)."""
         self.device = device
         self.keep_targets = keep_targets   # reuse idle contexts of requests' named maps
         self.hbm_budget = int(hbm_budget)
@@ -142,44 +159,55 @@ class FitServer:
         """Release every idle context (their memory returns to the budget)."""
         self._evict_idle(float("inf"))
 
+    def _batch_for(self, req):
+        if self.batch > 0:
+            return min(self.batch, req.rotations.shape[0])
+        return fit_batch(req.target.shape, req.rotations.shape[0], self.hbm_budget,
+                         self.max_active, self.mode)
+
     def _admit(self, now):
         while self.waiting and len(self.active) < self.max_active:
             req = self.waiting[0]
+            nrot = req.rotations.shape[0]
             held = self._take_idle(self._key(req))
             if held is not None:
                 corr, need = held           # its bytes are already counted
+                batch = corr.batch
+                grown = ctx_bytes(req.target.shape, batch, self.mode, 0, nrot)
+                if grown > need:            # a longer rotation set: count the difference
+                    self.used += grown - need
+                    need = grown
                 self.reused += 1
             else:
-                need = ctx_bytes(req.target.shape, self.batch, self.mode)
+                batch = self._batch_for(req)
+                need = ctx_bytes(req.target.shape, batch, self.mode, 0, nrot)
                 if need > self.hbm_budget:
                     raise MemoryError(f"request {req.rid} needs {need} bytes, over the budget")
                 self._evict_idle(need)
                 if self.used + need > self.hbm_budget:
                     break
                 corr = HipCorrelator(np.asarray(req.target, np.float64), laplace=req.laplace,
-                                     device=self.device, mode=self.mode, batch=self.batch)
+                                     device=self.device, mode=self.mode, batch=batch)
                 self.used += need
             self.waiting.pop(0)
             corr.template = req.template
             corr.mask = req.mask
             corr.rotations = req.rotations
             corr.reset()
-            self.active.append(_Active(req, corr, need, now))
+            self.active.append(_Active(req, corr, need, now, batch))
 
     def _pump(self):
-        """Queue one more batch on every active fit whose stream is not full."""
+        """Retire finished batches, then top every active fit up to `depth`
+        batches in flight."""
         for a in self.active:
+            while a.queued and a.corr.reached(a.queued[0]):
+                a.queued.pop(0)
             K = a.req.rotations.shape[0]
-            if a.next_rot >= K:
-                continue
-            if len(a.queued) >= self.depth:
-                if not a.corr.done():
-                    continue
-                a.queued.clear()
-            n = min(self.batch, K - a.next_rot)
-            a.corr.enqueue_scan(a.next_rot, n)
-            a.next_rot += n
-            a.queued.append(n)
+            while a.next_rot < K and len(a.queued) < self.depth:
+                n = min(a.batch, K - a.next_rot)
+                a.corr.enqueue_scan(a.next_rot, n)
+                a.next_rot += n
+                a.queued.append(a.corr.mark())
 
     def _complete(self, now_fn):
         still = []
@@ -224,25 +252,31 @@ class FitServer:
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
             elif self.active and all(a.next_rot >= a.req.rotations.shape[0] or
                                      len(a.queued) >= self.depth for a in self.active):
-                time.sleep(0.0002)
+                time.sleep(0.0001)
         return self.results
 
 
-def make_stream(n, rate, seed=5003, sizes=None, angles=(10.0, 20.0), maps=0):
-    """Poisson stream of synthetic fit requests (config: 500-20,000-atom chains
-    into 48^3-160^3 maps at 4-12 A, 10 or 20 degree rotation sets).  With
-    maps > 0 the requests draw from that many shared maps (target_key set), as
-    when several fits run against the same cryo-EM map."""
+def make_stream(n, rate, seed=5003, sizes=None, angles=(10.0, 20.0), maps=0,
+                max_rotations=None, core_weighted=()):
+    """Poisson stream of synthetic fit requests (500-20,000-atom chains into
+    48^3-160^3 maps at 4-12 A, 10 or 20 degree rotation sets; arrivals and
+    draws from `seed`).  With maps > 0 the requests draw from that many shared
+    maps (target_key set), as when several fits run against the same cryo-EM
+    map.  `sizes` as a list is drawn from; as a tuple of length n it is taken
+    in order.  max_rotations truncates each rotation set to a prefix; requests
+    whose index is in `core_weighted` use a core-weighted mask."""
     from .rotations import rotations_for_angle
     from .synth import Workload, make_case
     rng = np.random.default_rng(seed)
     sizes = sizes or [48, 64, 72, 80, 96, 120, 128, 144, 160]
+    fixed = isinstance(sizes, tuple) and len(sizes) == n
 
-    def draw_case(tag, k):
-        N = int(rng.choice(sizes))
+    def draw_case(tag, k, i=None):
+        N = int(sizes[i]) if fixed and i is not None else int(rng.choice(sizes))
         res = float(rng.uniform(4.0, 12.0))
         atoms = int(rng.integers(500, 20001))
-        return make_case(Workload(tag, N, res / 4.0, atoms, res, 10.0, seed=seed * 10 + k * 7))
+        return make_case(Workload(tag, N, res / 4.0, atoms, res, 10.0,
+                                  core_weighted=i in core_weighted, seed=seed * 10 + k * 7))
 
     shared = [draw_case(f"map{j}", 100000 + j) for j in range(maps)]
     t = 0.0
@@ -253,11 +287,30 @@ def make_stream(n, rate, seed=5003, sizes=None, angles=(10.0, 20.0), maps=0):
             j = int(rng.integers(0, maps))
             c, key = shared[j], f"map{j}"
         else:
-            c, key = draw_case(f"req{i}", i), None
+            c, key = draw_case(f"req{i}", i, i), None
         angle = float(rng.choice(angles))
-        reqs.append(FitRequest(i, c.target, c.template, c.mask, rotations_for_angle(angle),
-                               c.laplace, t, key))
+        R = rotations_for_angle(angle)
+        if max_rotations:
+            R = R[:max_rotations]
+        reqs.append(FitRequest(i, c.target, c.template, c.mask, R, c.laplace, t, key))
     return reqs
+
+
+def summarize(results, wall, rate):
+    """Throughput, p50/p99 latency overall and per grid size."""
+    res = list(results.values())
+    lat = np.array([r.latency for r in res])
+    rots = sum(r.rotations for r in res)
+    per_size = {}
+    for n in sorted({r.shape[0] for r in res}):
+        ls = np.array([r.latency for r in res if r.shape[0] == n])
+        per_size[str(n)] = {"requests": int(ls.size), "p50_s": float(np.percentile(ls, 50)),
+                            "p99_s": float(np.percentile(ls, 99))}
+    return {"requests": len(res), "wall_s": wall, "rotations": rots,
+            "rotations_per_s": rots / wall, "requests_per_s": len(res) / wall,
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)),
+            "per_size": per_size, "arrival_rate": rate}
 
 
 def main(argv=None):
@@ -265,9 +318,13 @@ def main(argv=None):
     ap.add_argument("--requests", type=int, default=12)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=5003)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=0,
+                    help="rotations per launch (0: per fit, up to 256 within its budget share)")
+    ap.add_argument("--depth", type=int, default=2, help="batches in flight per fit")
     ap.add_argument("--max-active", type=int, default=8)
     ap.add_argument("--budget-gib", type=float, default=64.0)
+    ap.add_argument("--max-rotations", type=int, default=0,
+                    help="truncate every rotation set to this prefix (0: whole sets)")
     ap.add_argument("--maps", type=int, default=0,
                     help="draw the requests from this many shared maps (0: one map per request)")
     ap.add_argument("--no-keep-targets", action="store_true",
@@ -275,9 +332,11 @@ def main(argv=None):
     ap.add_argument("--detail", action="store_true",
                     help="also print one JSON line per request (size, rotations, times)")
     args = ap.parse_args(argv)
-    reqs = make_stream(args.requests, args.rate, args.seed, maps=args.maps)
+    reqs = make_stream(args.requests, args.rate, args.seed, maps=args.maps,
+                       max_rotations=args.max_rotations or None)
     server = FitServer(hbm_budget=int(args.budget_gib * (1 << 30)), max_active=args.max_active,
-                       batch=args.batch, keep_targets=not args.no_keep_targets)
+                       batch=args.batch, depth=args.depth,
+                       keep_targets=not args.no_keep_targets)
     t = time.perf_counter()
     res = server.run(reqs)
     wall = time.perf_counter() - t
@@ -287,14 +346,10 @@ def main(argv=None):
                               "arrival_s": round(r.arrival, 4),
                               "queued_s": round(r.admitted - r.arrival, 4),
                               "latency_s": round(r.latency, 4)}))
-    lat = np.array([r.latency for r in res.values()])
-    rots = sum(r.rotations for r in res.values())
-    print(json.dumps({
-        "requests": len(res), "wall_s": wall, "rotations": rots,
-        "rotations_per_s": rots / wall, "requests_per_s": len(res) / wall,
-        "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
-        "sizes": sorted({r.shape[0] for r in res.values()}), "arrival_rate": args.rate,
-        "maps": args.maps, "contexts_reused": server.reused}))
+    out = summarize(res, wall, args.rate)
+    out.update({"maps": args.maps, "contexts_reused": server.reused, "depth": args.depth,
+                "batch": args.batch or "per fit"})
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

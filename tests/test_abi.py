@@ -33,12 +33,39 @@ def test_argument_errors_need_no_device():
 
 
 def test_ctx_bytes_model():
+    """Closed form of the admission estimate: resident inputs and keys,
+    rotations, and the larger of the scan's work buffers and the transient
+    buffers of pf_set_target (16 bytes per voxel)."""
     lib = _lib.load()
-    fused = lib.pf_ctx_bytes(128, 128, 128, 64, _lib.MODES["fused"])
-    unfused = lib.pf_ctx_bytes(128, 128, 128, 64, _lib.MODES["batched"])
-    plain = lib.pf_ctx_bytes(128, 128, 128, 64, _lib.MODES["plain"])
-    assert 0 < plain < fused < unfused
-    assert lib.pf_ctx_bytes(0, 1, 1, 1, 0) == -1
+    n, b, K = 128, 64, 7416
+    V, H = n ** 3, n * n * (n // 2 + 1)
+    resident = V * 25 + 2 * H * 8 + K * 72 + 4 * (((n // 8) ** 3 + 31) // 32)
+    fused = lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["fused"], 0, K)
+    unfused = lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["batched"], 0, K)
+    plain = lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["plain"], 0, K)
+    split = lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["auto"], _lib.FLAG_SPLIT_DFT, K)
+    assert fused == resident + 3 * b * H * 8
+    assert unfused == split == resident + 3 * b * (H * 8 + V * 4)
+    assert plain == resident + max(3 * (H * 8 + V * 4), 16 * V)
+    # a range shorter than the batch uses only as many rotations per launch
+    assert lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["fused"], 0, 3) == \
+        V * 25 + 2 * H * 8 + 3 * 72 + 4 * (((n // 8) ** 3 + 31) // 32) + 3 * 3 * H * 8
+    assert lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["fused"], 0, 1) == \
+        V * 25 + 2 * H * 8 + 72 + 4 * (((n // 8) ** 3 + 31) // 32) + 16 * V
+    # grids without a fused pipeline take the unfused work layout in auto mode
+    n2 = 120 + 6
+    V2, H2 = n2 ** 3, n2 * n2 * (n2 // 2 + 1)
+    assert lib.pf_ctx_bytes(n2, n2, n2, b, _lib.MODES["auto"], 0, 0) == \
+        V2 * 25 + 2 * H2 * 8 + 3 * b * (H2 * 8 + V2 * 4)
+    assert lib.pf_ctx_bytes(0, 1, 1, 1, 0, 0, 0) == -1
+    assert lib.pf_ctx_bytes(8, 8, 8, 1, 0, 0, -1) == -1
+
+
+def test_progress_marks_reject_bad_arguments():
+    lib = _lib.load()
+    t = ctypes.c_uint64(0)
+    assert lib.pf_ctx_mark(None, ctypes.byref(t)) == -1
+    assert lib.pf_ctx_reached(None, 1) == -1
 
 
 def test_more_argument_errors_need_no_device():

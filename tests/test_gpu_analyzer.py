@@ -88,3 +88,21 @@ def test_analyzer_on_search_result(tmp_path):
     assert (tmp_path / "a.out").read_text() == (tmp_path / "r.out").read_text()
     a.steps = 3
     assert a._positions == watershed_positions(lcc, 3)
+
+
+def test_analyzer_equals_reference_fixture(tmp_path):
+    """The whole Analyzer (GPU watershed, rows, file) on the map of the golden
+    fixture written by the reference's own Analyzer
+    (tests/golden/make_output_fixtures.py)."""
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    f = np.load(os.path.join(gold, "analyzer_ref.npz"))
+    print("analyzer on the reference fixture map", flush=True)
+    a = Analyzer(f["corr"], f["rotmat"], f["rot_ind"], steps=int(f["steps"]),
+                 voxelspacing=float(f["voxelspacing"]),
+                 origin=tuple(float(v) for v in f["origin"]), z_sigma=float(f["z_sigma"]))
+    assert a._positions == set(tuple(int(v) for v in p) for p in f["positions"])
+    assert np.array_equal(np.array([[float(v) for v in r] for r in a.solutions]), f["solutions"])
+    a.tofile(str(tmp_path / "solutions.out"))
+    with open(os.path.join(gold, "analyzer_ref_solutions.out")) as fh:
+        assert (tmp_path / "solutions.out").read_text() == fh.read()

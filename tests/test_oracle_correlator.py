@@ -135,3 +135,39 @@ def test_core_indices_closed_form():
     zz, yy, xx = np.indices(box.shape)
     expect = np.minimum.reduce([zz + 1, 7 - zz, yy + 1, 9 - yy, xx + 1, 11 - xx])
     assert np.array_equal(d, expect)
+
+
+def test_zero_variance_case_divides_zero_by_zero():
+    """The constructed zero-variance case really reaches calc_lcc with
+    G = 0 and variance 0 at every voxel (so take-best sees NaN), and the scan
+    keeps the initial (0, 0): NaN never wins (powerfitter.py:323-330)."""
+    from special_cases import cube_group, zero_variance_case
+    target, template, mask = zero_variance_case(32)
+    R, _ = cube_group()
+    assert R.shape[0] == 24
+    T, lcc_mask, rmax, Tf, F1, F2 = oc.prepare_target(target)
+    P, W, N = oc.prepare_template(template, mask)
+    assert lcc_mask.all() and N % 2 == 0
+    assert np.array_equal(P, template)          # normalisation leaves +-1 unchanged
+    for k in range(R.shape[0]):
+        s = oc.lcc_for_rotation(P, W, N, R[k], rmax, F1, F2, lcc_mask)
+        assert np.isnan(s).all(), k
+    res = oc.scan(target, template, mask, R)
+    assert not res.best.any() and not res.rot.any()
+
+
+def test_duplicate_rotations_keep_the_first_index():
+    """A rotation set listed twice: every per-rotation map of the second copy
+    ties exactly with the first, and strict '>' keeps the lower index
+    (powerfitter.py:323-330), also across the chunked merge (:146-163)."""
+    from powerfit_amd.rotations import rotations_for_angle
+    from powerfit_amd.synth import Workload, make_case
+    c = make_case(Workload("dup", 16, 3.0, 60, 8.0, 20.0, seed=7010))
+    R = rotations_for_angle(20.0)[:6]
+    one = oc.scan(c.target, c.template, c.mask, R)
+    two = oc.scan(c.target, c.template, c.mask, np.concatenate([R, R]))
+    assert np.array_equal(one.best, two.best) and np.array_equal(one.rot, two.rot)
+    inter = oc.scan(c.target, c.template, c.mask, np.repeat(R, 2, axis=0))
+    assert np.array_equal(inter.best, one.best) and np.array_equal(inter.rot, 2 * one.rot)
+    par = oc.parallel_scan(c.target, c.template, c.mask, np.concatenate([R, R]), nproc=3)
+    assert np.array_equal(par.best, one.best) and np.array_equal(par.rot, one.rot)
