@@ -73,9 +73,14 @@ __global__ void k_r2c_x(const float* __restrict__ in, float2* __restrict__ out, 
         const int f = u / nxh, k = u - f * nxh;
         const float2* z = Z + f * M;
         float2 X;
-        if (even) {
-            const float2 zk = z[k % M];
-            const float2 zm = cconj(z[(M - k) % M]);
+        if (even && (k == 0 || k == M)) {
+            // DC and Nyquist are real: the sum and the difference of the even
+            // and odd sums, with an imaginary part of exactly 0 as numpy's
+            const float2 z0 = z[0];
+            X = c2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.f);
+        } else if (even) {
+            const float2 zk = z[k];
+            const float2 zm = cconj(z[M - k]);
             const float2 E = cscale(cadd(zk, zm), 0.5f);
             const float2 O = mul_si<-1>(cscale(csub(zk, zm), 0.5f));
             X = cadd(E, cmul(twfull[k], O));
@@ -330,10 +335,24 @@ int twiddles(int n, const float2** out) {
         *out = it->second;
         return kOk;
     }
+    // exp(-2 pi i m / n) = (-i)^q exp(-i (pi/2) r / n) with 4 m = q n + r,
+    // 0 <= r < n: the quarter-turn points come out exactly (1, 0), (0, -1),
+    // (-1, 0), (0, 1), as in numpy's FFT, instead of carrying the rounding of
+    // cos and sin at multiples of pi / 2
     std::vector<float2> h(n);
     for (int m = 0; m < n; ++m) {
-        const double a = -2.0 * M_PI * (double)m / (double)n;
-        h[m] = c2((float)cos(a), (float)sin(a));
+        const int64_t m4 = 4LL * m;
+        const int q = (int)(m4 / n), r = (int)(m4 % n);
+        const double th = 0.5 * M_PI * (double)r / (double)n;
+        const double c = r == 0 ? 1.0 : cos(th), s = r == 0 ? 0.0 : sin(th);
+        double re, im;   // (c, -s) times (-i)^q
+        switch (q) {
+            case 0: re = c; im = -s; break;
+            case 1: re = -s; im = -c; break;
+            case 2: re = -c; im = s; break;
+            default: re = s; im = c; break;
+        }
+        h[m] = c2((float)(re + 0.0), (float)(im + 0.0));   // no negative zeros
     }
     float2* d = nullptr;
     PF_HIP_CHECK(hipMalloc(&d, sizeof(float2) * n));
