@@ -15,6 +15,7 @@
 #ifndef POWERFIT_HIP_H
 #define POWERFIT_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -127,6 +128,22 @@ int pf_scan(pf_ctx *ctx, int64_t first, int64_t count, int batch, int mode);
  * graph (its launches name those buffers and values); the next call captures
  * again.  Enqueues only. */
 int pf_scan_graph(pf_ctx *ctx, int64_t first, int64_t count, int batch, int mode);
+
+/* pf_scan that allocates nothing: the per-batch work buffers live in a static
+ * device array of PF_SCRATCH_BYTES per GPU (each GPU the library runs on
+ * holds it until the process ends) instead of ctx allocations, so the call
+ * makes no hipMalloc/hipFree and records no timing events.  No counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, modes and results as pf_scan; rejects with PF_ERR_ARG a batch
+ * whose work buffers exceed the scratch: 24 H bytes per rotation fused, plus
+ * 12 V bytes with an unfused stage (at 128^3 fused: up to 5 rotations per
+ * launch).  Every call on one GPU shares that GPU's scratch, whatever its ctx
+ * or stream, and the scratch keeps the last call's contents (no call reads a
+ * region it has not written): calls that overlap on one GPU race, so callers
+ * serialise them (one stream, or wait for the previous call).  Enqueues only. */
+#define PF_SCRATCH_BYTES ((size_t)128 << 20)
+int pf_scan_noalloc(pf_ctx *ctx, int64_t first, int64_t count, int batch, int mode);
 
 /* Copy out best LCC (float32) and rotation index (int32), shape (nz, ny, nx)
  * — GPUCorrelator's lcc/rot (:536-537).  Synchronises the ctx stream. */

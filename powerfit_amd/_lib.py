@@ -47,6 +47,7 @@ _SIGNATURES = {
     "pf_reset_result": ([_vp], _c_int),
     "pf_scan": ([_vp, _c_i64, _c_i64, _c_int, _c_int], _c_int),
     "pf_scan_graph": ([_vp, _c_i64, _c_i64, _c_int, _c_int], _c_int),
+    "pf_scan_noalloc": ([_vp, _c_i64, _c_i64, _c_int, _c_int], _c_int),
     "pf_get_result": ([_vp, _fp, _i32p], _c_int),
     "pf_sync": ([_vp], _c_int),
     "pf_ctx_profile": ([_vp, _c_int], _c_int),

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <deque>
+#include <mutex>
 #include <vector>
 
 #include "../../include/powerfit_hip.h"
@@ -11,6 +12,13 @@
 #include "pf_rotate.h"
 
 using namespace pf;
+
+// Work buffers of pf_scan_noalloc: one array per GPU, held from the library's
+// first use of that GPU until the process ends.  Host code never uses the
+// name as an address: the device address of each GPU's copy comes from
+// hipGetSymbolAddress (scratch_ptr below).
+constexpr size_t kScratchBytes = PF_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 struct pf_ctx {
     int device = 0;
@@ -184,6 +192,175 @@ int ensure_capacity(pf_ctx* c, int batch, bool need_real) {
     return kOk;
 }
 
+// Scan request after validation: mode resolved (AUTO), batch clamped to the
+// range, and which of the three stages run fused.
+struct ScanPlan {
+    int batch = 0, mode = 0;
+    bool fa = false, fb = false, fc = false;   // fused forward / z / inverse stage
+    bool need_real = false;                     // real work grids (an unfused stage)
+};
+
+int plan_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode, ScanPlan* sp) {
+    if (!c) return arg_error("ctx is null");
+    if (!c->have_target || !c->have_template || c->nrot == 0) {
+        set_error("First set the template, mask, and rotations.");
+        return kErrState;
+    }
+    if (first < 0 || count < 0 || first + count > c->nrot)
+        return arg_error("rotation range outside the rotation set");
+    if (mode < PF_MODE_PLAIN || mode > PF_MODE_MIX_INV) return arg_error("unknown mode");
+    // the split-precision FFT stage exists in the unfused passes only
+    const bool split = (c->flags & PF_FLAG_SPLIT_DFT) != 0;
+    const bool fused_ok =
+        !split && fused_supported(c->nz, c->ny, c->nx) && c->rmax <= c->nx / 2;
+    if (mode == PF_MODE_AUTO) mode = fused_ok ? PF_MODE_FUSED : PF_MODE_BATCHED;
+    if (mode >= PF_MODE_FUSED && !fused_ok) {
+        set_error(split ? "fused pipeline not available with PF_FLAG_SPLIT_DFT"
+                        : "fused pipeline not available for grid (%d, %d, %d)",
+                  c->nz, c->ny, c->nx);
+        return kErrUnsupported;
+    }
+    if (mode == PF_MODE_PLAIN) batch = 1;
+    if (batch < 1 || batch > 4096) return arg_error("batch must be in [1, 4096]");
+    // a shorter range needs no more work buffer than its own length (results
+    // do not depend on the batch size)
+    if (count > 0 && batch > count) batch = (int)count;
+    sp->batch = batch;
+    sp->mode = mode;
+    // stage A: rotate + forward xy; B: z correlate; C: inverse xy + LCC
+    sp->fa = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_FWD;
+    sp->fb = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_Z;
+    sp->fc = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_INV;
+    sp->need_real = !(sp->fa && sp->fc);
+    return kOk;
+}
+
+// Enqueue the launches of rotations [first, first + count) on the ctx stream
+// with the given work buffers (cplx: 3 * batch * H complex; real: 3 * batch * V
+// floats when an unfused stage runs).  Every region of the work buffers a
+// launch reads was written earlier in the same batch, so they need no
+// clearing.  With `timing`, launches are bracketed by the ctx's profiling
+// events when profiling is on.
+int enqueue_scan(pf_ctx* c, int64_t first, int64_t count, const ScanPlan& sp, float2* cplx,
+                 float* real, bool timing) {
+    const int batch = sp.batch;
+    const bool fa = sp.fa, fb = sp.fb, fc = sp.fc;
+    auto run = [&](int which, auto&& f) -> int { return timing ? timed(c, which, f) : f(); };
+    const double invV = 1.0 / (double)c->V;
+    const int64_t plane = (int64_t)c->ny * c->fft.nxh;
+    FusedArgs fa_args{};
+    fa_args.n = c->nx;
+    fa_args.tm = c->d_tm;
+    fa_args.occ = c->d_occ;
+    fa_args.rmax = c->rmax;
+    fa_args.zr = c->support_zr;
+    fa_args.zsched = c->zsched;
+    fa_args.skip_planes = fa && fb;
+    fa_args.work = cplx;
+    fa_args.F1 = c->d_F1;
+    fa_args.F2 = c->d_F2;
+    fa_args.lcc_mask = c->d_lccmask;
+    fa_args.keys = c->d_keys;
+    fa_args.norm = c->norm;
+    fa_args.invV = invV;
+    // enough blocks: z kernel ~ ceil(H / (N * 32)) tiles per chunk, LCC kernel N planes
+    {
+        const int64_t tiles = (c->H / c->nz + 31) / 32;
+        const int64_t want = 4LL * c->num_cu;
+        const int bz = (int)std::max<int64_t>(1, std::min<int64_t>(batch, tiles * batch / want));
+        const int bl = (int)std::max<int64_t>(1, std::min<int64_t>(batch, (int64_t)c->nz * batch / want));
+        fa_args.chunk_z = c->chunk_z > 0 ? c->chunk_z : std::min(bz, 16);
+        fa_args.chunk_lcc = c->chunk_lcc > 0 ? c->chunk_lcc : std::min(bl, 255);
+    }
+    for (int64_t k0 = first; k0 < first + count; k0 += batch) {
+        const int nb = (int)std::min<int64_t>(batch, first + count - k0);
+        fa_args.rots = c->d_rot + 9 * k0;
+        fa_args.nrot = nb;
+        fa_args.first = k0;
+        if (fa) {
+            // W^2 = W shortcut only when the z stage is fused (it fills grid 2)
+            fa_args.binary_mask = c->mask_binary && fb;
+            PF_CHECK_STATUS(run(PF_K_FWD_PLANE, [&] { return launch_fused(fa_args, 0, c->stream); }));
+        } else {
+            PF_CHECK_STATUS(run(PF_K_ROTATE, [&] {
+                return launch_rotate_pw(c->d_template, c->d_mask, c->d_rot + 9 * k0, nb, c->nz,
+                                        c->ny, c->nx, c->rmax, real, c->stream);
+            }));
+            PF_CHECK_STATUS(run(PF_K_R2C_X, [&] {
+                return launch_r2c_x(c->fft, real, cplx, (int64_t)3 * nb * c->nz * c->ny,
+                                    c->stream);
+            }));
+            PF_CHECK_STATUS(run(PF_K_C2C_FWD, [&] {
+                return launch_axis(c->fft, 1, cplx, c->fft.nxh, c->fft.nxh,
+                                   (int64_t)3 * nb * c->nz, plane, -1, c->stream);
+            }));
+        }
+        if (fb) {
+            fa_args.binary_mask = c->mask_binary;
+            PF_CHECK_STATUS(run(PF_K_ZCORR, [&] { return launch_fused(fa_args, 1, c->stream); }));
+        } else {
+            PF_CHECK_STATUS(run(PF_K_C2C_FWD, [&] {
+                return launch_axis(c->fft, 2, cplx, plane, plane, (int64_t)3 * nb,
+                                   plane * c->nz, -1, c->stream);
+            }));
+            PF_CHECK_STATUS(run(PF_K_CONJ_MUL, [&] {
+                return launch_conj_mul_pw(cplx, c->d_F1, c->d_F2, c->H, nb, c->stream);
+            }));
+            PF_CHECK_STATUS(run(PF_K_C2C_INV, [&] {
+                return launch_axis(c->fft, 2, cplx, plane, plane, (int64_t)3 * nb,
+                                   plane * c->nz, 1, c->stream);
+            }));
+        }
+        if (fc) {
+            PF_CHECK_STATUS(run(PF_K_INV_PLANE, [&] { return launch_fused(fa_args, 2, c->stream); }));
+        } else {
+            PF_CHECK_STATUS(run(PF_K_C2C_INV, [&] {
+                return launch_axis(c->fft, 1, cplx, c->fft.nxh, c->fft.nxh,
+                                   (int64_t)3 * nb * c->nz, plane, 1, c->stream);
+            }));
+            PF_CHECK_STATUS(run(PF_K_C2R_X, [&] {
+                return launch_c2r_x(c->fft, cplx, real, (int64_t)3 * nb * c->nz * c->ny,
+                                    1.0f, c->stream);
+            }));
+            PF_CHECK_STATUS(run(PF_K_LCC, [&] {
+                return launch_lcc_take_best(real, c->d_lccmask, c->d_keys, c->V, nb, k0,
+                                            c->norm, invV, c->stream);
+            }));
+        }
+    }
+    return kOk;
+}
+
+// Per-device address of `scratch`, resolved once per device with that device
+// current (pf_ctx_create calls resolve_scratch for its device).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+int resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device %d beyond the %d the library supports", device, kMaxDevices);
+        return kErrArg;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return kOk;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        set_error("hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        return kErrHip;
+    }
+    g_scratch[device] = (unsigned char*)p;
+    return kOk;
+}
+
+unsigned char* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[device];
+}
+
 bool radius_ok(int r, int a, int b, int c) {
     return r >= 0 && r <= std::min(a, std::min(b, c)) - 2;
 }
@@ -284,6 +461,8 @@ int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** o
     c->H = (int64_t)nz * ny * (nx / 2 + 1);
     DeviceGuard g(device);
     int s = g.ok ? kOk : kErrHip;
+    // the scratch of pf_scan_noalloc on this device (with the device current)
+    if (!s) s = resolve_scratch(device);
     if (!s) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
@@ -466,119 +645,35 @@ int pf_reset_result(pf_ctx* c) {
 }
 
 int pf_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
-    if (!c) return arg_error("ctx is null");
-    if (!c->have_target || !c->have_template || c->nrot == 0) {
-        set_error("First set the template, mask, and rotations.");
+    ScanPlan sp;
+    PF_CHECK_STATUS(plan_scan(c, first, count, batch, mode, &sp));
+    PF_DEVICE_GUARD(c);
+    PF_CHECK_STATUS(ensure_capacity(c, sp.batch, sp.need_real));
+    return enqueue_scan(c, first, count, sp, c->d_cplx, c->d_real, true);
+}
+
+int pf_scan_noalloc(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) {
+    ScanPlan sp;
+    PF_CHECK_STATUS(plan_scan(c, first, count, batch, mode, &sp));
+    const size_t cplx = (size_t)3 * sp.batch * c->H * sizeof(float2);
+    const size_t real = sp.need_real ? (size_t)3 * sp.batch * c->V * sizeof(float) : 0;
+    // the real grids start at the first 256-byte boundary after the spectra
+    const size_t roff = (cplx + 255) & ~(size_t)255;
+    const size_t need = sp.need_real ? roff + real : cplx;
+    if (need > kScratchBytes) {
+        set_error("pf_scan_noalloc: batch %d of a (%d, %d, %d) grid needs %zu work bytes, "
+                  "more than the %zu-byte scratch; use a smaller batch",
+                  sp.batch, c->nz, c->ny, c->nx, need, kScratchBytes);
+        return kErrArg;
+    }
+    unsigned char* base = scratch_ptr(c->device);
+    if (!base) {
+        set_error("pf_scan_noalloc: no scratch address for device %d", c->device);
         return kErrState;
     }
-    if (first < 0 || count < 0 || first + count > c->nrot)
-        return arg_error("rotation range outside the rotation set");
-    if (mode < PF_MODE_PLAIN || mode > PF_MODE_MIX_INV) return arg_error("unknown mode");
-    // the split-precision FFT stage exists in the unfused passes only
-    const bool split = (c->flags & PF_FLAG_SPLIT_DFT) != 0;
-    const bool fused_ok =
-        !split && fused_supported(c->nz, c->ny, c->nx) && c->rmax <= c->nx / 2;
-    if (mode == PF_MODE_AUTO) mode = fused_ok ? PF_MODE_FUSED : PF_MODE_BATCHED;
-    if (mode >= PF_MODE_FUSED && !fused_ok) {
-        set_error(split ? "fused pipeline not available with PF_FLAG_SPLIT_DFT"
-                        : "fused pipeline not available for grid (%d, %d, %d)",
-                  c->nz, c->ny, c->nx);
-        return kErrUnsupported;
-    }
-    if (mode == PF_MODE_PLAIN) batch = 1;
-    if (batch < 1 || batch > 4096) return arg_error("batch must be in [1, 4096]");
-    // a shorter range needs no more work buffer than its own length (results
-    // do not depend on the batch size)
-    if (count > 0 && batch > count) batch = (int)count;
-    // stage A: rotate + forward xy; B: z correlate; C: inverse xy + LCC
-    const bool fa = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_FWD;
-    const bool fb = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_Z;
-    const bool fc = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_INV;
     PF_DEVICE_GUARD(c);
-    PF_CHECK_STATUS(ensure_capacity(c, batch, !(fa && fc)));
-    const double invV = 1.0 / (double)c->V;
-    const int64_t plane = (int64_t)c->ny * c->fft.nxh;
-    FusedArgs fa_args{};
-    fa_args.n = c->nx;
-    fa_args.tm = c->d_tm;
-    fa_args.occ = c->d_occ;
-    fa_args.rmax = c->rmax;
-    fa_args.zr = c->support_zr;
-    fa_args.zsched = c->zsched;
-    fa_args.skip_planes = fa && fb;
-    fa_args.work = c->d_cplx;
-    fa_args.F1 = c->d_F1;
-    fa_args.F2 = c->d_F2;
-    fa_args.lcc_mask = c->d_lccmask;
-    fa_args.keys = c->d_keys;
-    fa_args.norm = c->norm;
-    fa_args.invV = invV;
-    // enough blocks: z kernel ~ ceil(H / (N * 32)) tiles per chunk, LCC kernel N planes
-    {
-        const int64_t tiles = (c->H / c->nz + 31) / 32;
-        const int64_t want = 4LL * c->num_cu;
-        const int bz = (int)std::max<int64_t>(1, std::min<int64_t>(batch, tiles * batch / want));
-        const int bl = (int)std::max<int64_t>(1, std::min<int64_t>(batch, (int64_t)c->nz * batch / want));
-        fa_args.chunk_z = c->chunk_z > 0 ? c->chunk_z : std::min(bz, 16);
-        fa_args.chunk_lcc = c->chunk_lcc > 0 ? c->chunk_lcc : std::min(bl, 255);
-    }
-    for (int64_t k0 = first; k0 < first + count; k0 += batch) {
-        const int nb = (int)std::min<int64_t>(batch, first + count - k0);
-        fa_args.rots = c->d_rot + 9 * k0;
-        fa_args.nrot = nb;
-        fa_args.first = k0;
-        if (fa) {
-            // W^2 = W shortcut only when the z stage is fused (it fills grid 2)
-            fa_args.binary_mask = c->mask_binary && fb;
-            PF_CHECK_STATUS(timed(c, PF_K_FWD_PLANE, [&] { return launch_fused(fa_args, 0, c->stream); }));
-        } else {
-            PF_CHECK_STATUS(timed(c, PF_K_ROTATE, [&] {
-                return launch_rotate_pw(c->d_template, c->d_mask, c->d_rot + 9 * k0, nb, c->nz,
-                                        c->ny, c->nx, c->rmax, c->d_real, c->stream);
-            }));
-            PF_CHECK_STATUS(timed(c, PF_K_R2C_X, [&] {
-                return launch_r2c_x(c->fft, c->d_real, c->d_cplx, (int64_t)3 * nb * c->nz * c->ny,
-                                    c->stream);
-            }));
-            PF_CHECK_STATUS(timed(c, PF_K_C2C_FWD, [&] {
-                return launch_axis(c->fft, 1, c->d_cplx, c->fft.nxh, c->fft.nxh,
-                                   (int64_t)3 * nb * c->nz, plane, -1, c->stream);
-            }));
-        }
-        if (fb) {
-            fa_args.binary_mask = c->mask_binary;
-            PF_CHECK_STATUS(timed(c, PF_K_ZCORR, [&] { return launch_fused(fa_args, 1, c->stream); }));
-        } else {
-            PF_CHECK_STATUS(timed(c, PF_K_C2C_FWD, [&] {
-                return launch_axis(c->fft, 2, c->d_cplx, plane, plane, (int64_t)3 * nb,
-                                   plane * c->nz, -1, c->stream);
-            }));
-            PF_CHECK_STATUS(timed(c, PF_K_CONJ_MUL, [&] {
-                return launch_conj_mul_pw(c->d_cplx, c->d_F1, c->d_F2, c->H, nb, c->stream);
-            }));
-            PF_CHECK_STATUS(timed(c, PF_K_C2C_INV, [&] {
-                return launch_axis(c->fft, 2, c->d_cplx, plane, plane, (int64_t)3 * nb,
-                                   plane * c->nz, 1, c->stream);
-            }));
-        }
-        if (fc) {
-            PF_CHECK_STATUS(timed(c, PF_K_INV_PLANE, [&] { return launch_fused(fa_args, 2, c->stream); }));
-        } else {
-            PF_CHECK_STATUS(timed(c, PF_K_C2C_INV, [&] {
-                return launch_axis(c->fft, 1, c->d_cplx, c->fft.nxh, c->fft.nxh,
-                                   (int64_t)3 * nb * c->nz, plane, 1, c->stream);
-            }));
-            PF_CHECK_STATUS(timed(c, PF_K_C2R_X, [&] {
-                return launch_c2r_x(c->fft, c->d_cplx, c->d_real, (int64_t)3 * nb * c->nz * c->ny,
-                                    1.0f, c->stream);
-            }));
-            PF_CHECK_STATUS(timed(c, PF_K_LCC, [&] {
-                return launch_lcc_take_best(c->d_real, c->d_lccmask, c->d_keys, c->V, nb, k0,
-                                            c->norm, invV, c->stream);
-            }));
-        }
-    }
-    return kOk;
+    float* rbuf = sp.need_real ? (float*)(base + roff) : nullptr;
+    return enqueue_scan(c, first, count, sp, (float2*)base, rbuf, false);
 }
 
 int pf_ctx_query(pf_ctx* c) {
