@@ -187,6 +187,60 @@ def precision_report(args, HipCorrelator, case, R, w):
     return out
 
 
+def serving_bench(args):
+    """Serving configuration: a Poisson stream of mixed-size fits through
+    FitServer (throughput, p50/p99 latency overall and per grid size), then
+    the same stream with every rotation set cut to a prefix, served again,
+    each served map checked against the FP64 oracle on that prefix."""
+    import dataclasses
+
+    from powerfit_amd.serve import FitServer, make_stream, summarize
+
+    stream = make_stream(args.serve_requests, args.serve_rate, seed=5003)
+    print(f"# stream of {len(stream)} requests ready", file=sys.stderr, flush=True)
+
+    def serve(max_rot):
+        reqs = [dataclasses.replace(r, rotations=r.rotations[:max_rot] if max_rot else r.rotations)
+                for r in stream]
+        server = FitServer(max_active=args.serve_max_active, depth=args.serve_depth)
+        t = time.perf_counter()
+        res = server.run(reqs)
+        return reqs, res, time.perf_counter() - t, server
+
+    reqs, res, wall, server = serve(0)
+    out = {"metric": "serving: rotations/s and latency of a Poisson fit stream",
+           "config": {"workload": "serving: Poisson stream (seed 5003), 48-160 cubed maps, "
+                                  "500-20,000-atom chains at 4-12 A, 10 or 20 deg sets",
+                      "requests": args.serve_requests, "rate_per_s": args.serve_rate,
+                      "max_active": args.serve_max_active, "depth": args.serve_depth},
+           "summary": summarize(res, wall, args.serve_rate),
+           "requests": [{"request": r.rid, "n": r.shape[0], "rotations": r.rotations,
+                         "arrival_s": round(r.arrival, 4),
+                         "queued_s": round(r.admitted - r.arrival, 4),
+                         "latency_s": round(r.latency, 4)}
+                        for r in sorted(res.values(), key=lambda r: r.rid)]}
+    print(json.dumps(out), flush=True)
+    if args.serve_check > 0:
+        # checker: the FP64 oracle (the reference's CPU path) on each request's prefix
+        from oracle import correlator as oc
+        reqs, res, wall, _ = serve(args.serve_check)
+        rows = []
+        for r in reqs:
+            ref = oc.parallel_scan(np.asarray(r.target, np.float64), r.template, r.mask,
+                                   r.rotations, laplace=r.laplace, nproc=cpu_workers())
+            row = {"request": r.rid, "n": int(r.target.shape[0]),
+                   "rotations": int(r.rotations.shape[0])}
+            row.update(cpu_baseline_check(None, None, res[r.rid].lcc, res[r.rid].rot, ref=ref))
+            rows.append(row)
+            print(json.dumps(row), flush=True)
+        print(json.dumps({"serving_check": {
+            "prefix_rotations": args.serve_check, "requests": len(rows),
+            "max_abs_lcc_err": max(x["max_abs_lcc_err"] for x in rows),
+            "index_mismatches": sum(x["index_mismatches"] for x in rows),
+            "tolerance": 1e-4}}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,12 +274,25 @@ def main(argv=None):
     ap.add_argument("--precision-workload", default="prec192")
     ap.add_argument("--precision-rotations", type=int, default=32)
     ap.add_argument("--precision-batch", type=int, default=8)
+    ap.add_argument("--serving", action="store_true",
+                    help="measure the serving configuration (Poisson stream of mixed-size fits) "
+                         "instead of the search workload")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-max-active", type=int, default=8)
+    ap.add_argument("--serve-depth", type=int, default=2)
+    ap.add_argument("--serve-check", type=int, default=16,
+                    help="serve the stream again with every rotation set cut to this prefix and "
+                         "check each served map against the FP64 oracle (0 = skip)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         return 2
+
+    if args.serving:
+        return serving_bench(args)
 
     import torch
     from powerfit_amd import HipCorrelator, _lib

@@ -96,3 +96,43 @@ extern "C" int pf_host_group_fft_ex(int n, int t, int dir, const float* in, floa
 }
 
 extern "C" int pf_host_pick_T(int n, int emax, int tmax) { return pf::pick_T(n, emax, tmax); }
+
+// Row FFT with padded exchanges (PAD > 0): the transform of the group's
+// registers (to_ex = 0) or, for the TO_EX form, the natural-order result the
+// last stage leaves in the exchange buffer (to_ex = 1).
+template <int N, int T, int PAD, bool TO_EX>
+static int group_fft_pad_host(int dir, const float* in, float* out) {
+    constexpr int E = N / T;
+    std::vector<float2> tw(N), ex(pf::pad_extent(N, PAD), pf::c2(0.f, 0.f));
+    for (int m = 0; m < N; ++m) {
+        const double ang = -2.0 * M_PI * (double)m / (double)N;
+        tw[m] = pf::c2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    std::vector<float2> regs((size_t)T * E);
+    auto* v = reinterpret_cast<float2(*)[E]>(regs.data());
+    for (int t = 0; t < T; ++t)
+        for (int e = 0; e < E; ++e) v[t][e] = pf::c2(in[2 * (t + T * e)], in[2 * (t + T * e) + 1]);
+    if (dir < 0) pf::GroupStage<N, T, -1, 0, false, TO_EX, PAD>::emulate(v, ex.data(), tw.data());
+    else pf::GroupStage<N, T, 1, 0, false, TO_EX, PAD>::emulate(v, ex.data(), tw.data());
+    for (int t = 0; t < T; ++t)
+        for (int e = 0; e < E; ++e) {
+            const int j = t + T * e;
+            const float2 r = TO_EX ? ex[j] : v[t][e];
+            out[2 * j] = r.x;
+            out[2 * j + 1] = r.y;
+        }
+    return (int)ex.size();
+}
+
+#define PF_PAD_CASE(N, T, PAD)                                                              \
+    if (n == N && t == T && pad == PAD)                                                     \
+        return to_ex ? group_fft_pad_host<N, T, PAD, true>(dir, in, out)                    \
+                     : group_fft_pad_host<N, T, PAD, false>(dir, in, out);
+
+extern "C" int pf_host_group_fft_pad(int n, int t, int pad, int to_ex, int dir, const float* in,
+                                     float* out) {
+    PF_PAD_CASE(32, 4, 8) PF_PAD_CASE(48, 4, 4) PF_PAD_CASE(64, 8, 8) PF_PAD_CASE(80, 4, 8)
+    PF_PAD_CASE(96, 4, 16) PF_PAD_CASE(96, 8, 16)
+    return -1;
+}
+

@@ -43,6 +43,21 @@ constexpr int padded_stride(int M) {
     return s;
 }
 
+// Padding period of the row FFTs' LDS exchanges (0 = none) for M-point rows
+// run by T threads: one free slot after every PAD elements spreads a stage's
+// stride-R butterfly writes, and the rows a wavefront covers, over the banks
+// (chosen with an LDS bank model of every stage's accesses; the padded row
+// fits the row stride: pad_extent(M, PAD) <= padded_stride(M + 1)).  Measured
+// per rotation: 128^3 K3 11.0 -> 10.4 us; 64^3 and 96^3 unchanged; 192^3 K3
+// (M = 96, PAD 16) 76.4 -> 79.9 us, so that size keeps the plain layout.
+constexpr int row_pad(int M, int T) {
+    return (M == 32 && T == 4) ? 8
+         : (M == 48 && T == 4) ? 4
+         : (M == 64 && T == 8) ? 8
+         : (M == 80 && T == 4) ? 8
+         : 0;
+}
+
 // Row FFTs (groups of consecutive lanes, one wavefront) synchronise their
 // stages per wavefront.  Column FFTs run lanes across columns (conflict-free
 // LDS, coalesced loads) with workgroup barriers, or, with kWaveCols, each
@@ -181,6 +196,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     constexpr int EY = N / TY;
     constexpr int CPI = NTHR / TY;
     constexpr bool WROWS = kWaveRows && (64 % TX == 0);
+    constexpr int XPAD = row_pad(M, TX);
+    static_assert(pad_extent(M, XPAD) <= G::S, "padded row fits the row stride");
     constexpr bool WCOLS = kWaveCols && (64 % TY == 0);
     static_assert(TX > 0 && NTHR % TX == 0, "row plan");
     static_assert(TY > 0 && NTHR % TY == 0 && CPI >= M, "one column pass");
@@ -278,7 +295,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
             }
             float2* rowp = plane + row * G::S;
             // row FFT; its result stays in the row (natural order)
-            group_fft_ex<M, TX, -1, WROWS>(v, active ? rowp : plane, 1, tx, stwM, active);
+            group_fft_ex<M, TX, -1, WROWS, XPAD>(v, active ? rowp : plane, 1, tx, stwM, active);
         }
         __syncthreads();
         {
@@ -420,6 +437,8 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
     constexpr int CPI = NTHR / TY;
     constexpr int CSW = (M + CPI - 1) / CPI;
     constexpr bool WROWS = kWaveRows && (64 % TX == 0);
+    constexpr int XPAD = row_pad(M, TX);
+    static_assert(pad_extent(M, XPAD) <= G::S, "padded row fits the row stride");
     constexpr bool WCOLS = kWaveCols && (64 % TY == 0);
     static_assert(TY > 0 && NTHR % TY == 0, "column plan");
     static_assert(NTHR >= N, "one Nyquist element per thread");
@@ -544,7 +563,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
             }
             // the row's own threads read it above and overwrite it below
             group_sync<WROWS>();
-            group_fft<M, TX, 1, WROWS>(vr, rowp, 1, tx, stwM, active);
+            group_fft<M, TX, 1, WROWS, XPAD>(vr, rowp, 1, tx, stwM, active);
             if constexpr (ISDG) {
                 if (!active) continue;
                 // B, then 1/sqrt(var), of voxels x = 2m, 2m + 1 in the B plane's region
@@ -643,6 +662,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane2(const float2* __restrict__ 
                   EX = P::EX;
     constexpr int OCCW = (G::NB * G::NB * G::NB + 31) / 32;
     constexpr bool WROWS = kWaveRows && (64 % TX == 0);
+    constexpr int XPAD = row_pad(M, TX);
+    static_assert(pad_extent(M, XPAD) <= G::S, "padded row fits the row stride");
     static_assert(P::ok, "pair plan");
     __shared__ float2 planes[2 * N * S];
     __shared__ uint32_t occ[OCCW];
@@ -693,8 +714,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane2(const float2* __restrict__ 
         }
         float2* rowP = active ? planes + row * S : planes;
         float2* rowW = active ? planes + N * S + row * S : planes;
-        group_fft_ex<M, TX, -1, WROWS>(vp, rowP, 1, tx, stwM, active);
-        group_fft_ex<M, TX, -1, WROWS>(vw, rowW, 1, tx, stwM, active);
+        group_fft_ex<M, TX, -1, WROWS, XPAD>(vp, rowP, 1, tx, stwM, active);
+        group_fft_ex<M, TX, -1, WROWS, XPAD>(vw, rowW, 1, tx, stwM, active);
     }
     __syncthreads();
     {
@@ -807,6 +828,8 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
     constexpr int TYS = pick_T_cols(N, M, NTHR, EMAX);             // one-plane layout
     constexpr int EYS = N / TYS, CPIS = NTHR / TYS;
     constexpr bool WROWS = kWaveRows && (64 % TX == 0);
+    constexpr int XPAD = row_pad(M, TX);
+    static_assert(pad_extent(M, XPAD) <= G::S, "padded row fits the row stride");
     static_assert(P::ok, "pair plan");
     static_assert(TYS > 0 && NTHR % TYS == 0 && CPIS >= M, "one column pass");
     __shared__ float2 planes[2 * N * S];
@@ -861,7 +884,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
         }
         // the row's own threads read it above and overwrite it below
         group_sync<WROWS>();
-        group_fft<M, TX, 1, WROWS>(vr, rowp, 1, tx, stwM, ract);
+        group_fft<M, TX, 1, WROWS, XPAD>(vr, rowp, 1, tx, stwM, ract);
     };
 
     if (b0 < b1) {

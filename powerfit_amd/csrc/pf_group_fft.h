@@ -224,11 +224,28 @@ __device__ __forceinline__ void group_sync() {
 }
 #endif
 
-template <int N, int T, int DIR, int S, bool WAVE = false, bool TO_EX = false>
+// Exchange slots an N-point transform occupies when PAD > 0 leaves one unused
+// slot after every PAD elements (element p at p + p / PAD).
+constexpr int pad_extent(int N, int PAD) { return PAD > 0 ? N + (N - 1) / PAD : N; }
+
+// PAD > 0 (contiguous exchanges, es = 1, e.g. the rows of a plane): element p
+// of every exchange but the last stage's output of a TO_EX transform sits at
+// p + p / PAD.  A stage's butterflies write their R outputs at stride R per
+// thread, so without padding the threads of one row, and the rows of one
+// wavefront (row stride = 8 mod 32 complex), fall on a few LDS banks; with
+// PAD = R the writes spread over all banks.  Needs pad_extent(N, PAD) slots.
+template <int N, int T, int DIR, int S, bool WAVE = false, bool TO_EX = false, int PAD = 0>
 struct GroupStage {
     static constexpr GPlan P = gplan_for(N, T);
     static constexpr int E = N / T;
     static constexpr bool kLast = S >= P.ns;
+    // the last stage of a TO_EX transform leaves its outputs in natural order
+    static constexpr bool kNatural = TO_EX && S + 1 >= P.ns;
+
+    static PF_HD int xpos(int p) {
+        if constexpr (PAD > 0) return p + p / PAD;
+        else return p;
+    }
 
     static constexpr int span() {
         int ns = 1;
@@ -260,7 +277,8 @@ struct GroupStage {
                 dft_any<R, DIR>(u);
                 const int od = (j - k) * R + k;
 #pragma unroll
-                for (int r = 0; r < R; ++r) ex[(od + r * Ns) * es] = u[r];
+                for (int r = 0; r < R; ++r)
+                    ex[(kNatural ? od + r * Ns : xpos(od + r * Ns)) * es] = u[r];
             }
         }
     }
@@ -268,7 +286,7 @@ struct GroupStage {
     // Next stage's inputs (strided layout) back from ex.
     static PF_HD void read_phase(float2 (&v)[N / T], const float2* ex, int es, int t) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = ex[(t + T * e) * es];
+        for (int e = 0; e < E; ++e) v[e] = ex[xpos(t + T * e) * es];
     }
 
 #if defined(__HIP__)
@@ -282,7 +300,7 @@ struct GroupStage {
             group_sync<WAVE>();
             if (active) read_phase(v, ex, es, t);
             group_sync<WAVE>();
-            GroupStage<N, T, DIR, S + 1, WAVE, TO_EX>::run(v, ex, es, t, tw, active);
+            GroupStage<N, T, DIR, S + 1, WAVE, TO_EX, PAD>::run(v, ex, es, t, tw, active);
         }
     }
 #endif
@@ -292,7 +310,7 @@ struct GroupStage {
         if constexpr (!kLast) {
             for (int t = 0; t < T; ++t) write_phase(v[t], ex, 1, t, tw);
             for (int t = 0; t < T; ++t) read_phase(v[t], ex, 1, t);
-            GroupStage<N, T, DIR, S + 1, WAVE, TO_EX>::emulate(v, ex, tw);
+            GroupStage<N, T, DIR, S + 1, WAVE, TO_EX, PAD>::emulate(v, ex, tw);
         }
     }
 };
@@ -302,24 +320,24 @@ struct GroupStage {
 // ex[0 .. (N-1) * es] (N elements with element stride es).  WAVE = true
 // requires every group's T threads to lie in one wavefront (T divides 64 and
 // groups are formed from consecutive lanes).
-template <int N, int T, int DIR, bool WAVE = false>
+template <int N, int T, int DIR, bool WAVE = false, int PAD = 0>
 __device__ __forceinline__ void group_fft(float2 (&v)[N / T], float2* ex, int es, int t,
                                           const float2* __restrict__ tw, bool active) {
     static_assert(gplan_for(N, T).ns >= 0, "no radix plan for this N and T");
     static_assert(!WAVE || (64 % T == 0), "wave-local groups need T | 64");
-    GroupStage<N, T, DIR, 0, WAVE>::run(v, ex, es, t, tw, active);
+    GroupStage<N, T, DIR, 0, WAVE, false, PAD>::run(v, ex, es, t, tw, active);
 }
 
 // As group_fft, but the transform is left in ex in natural order (element j
 // at ex[j * es]) and v holds no defined result.  The last stage ends without
 // a synchronisation: readers of ex from other threads must synchronise first
 // (a workgroup barrier, or group_sync<true> when they are in the same wave).
-template <int N, int T, int DIR, bool WAVE = false>
+template <int N, int T, int DIR, bool WAVE = false, int PAD = 0>
 __device__ __forceinline__ void group_fft_ex(float2 (&v)[N / T], float2* ex, int es, int t,
                                              const float2* __restrict__ tw, bool active) {
     static_assert(gplan_for(N, T).ns >= 1, "no radix plan for this N and T");
     static_assert(!WAVE || (64 % T == 0), "wave-local groups need T | 64");
-    GroupStage<N, T, DIR, 0, WAVE, true>::run(v, ex, es, t, tw, active);
+    GroupStage<N, T, DIR, 0, WAVE, true, PAD>::run(v, ex, es, t, tw, active);
 }
 #endif
 

@@ -59,3 +59,22 @@ def test_group_fft_matches_numpy(host, n, t):
             ns = fn(n, t, d, x.ctypes.data_as(FP), out.ctypes.data_as(FP))
             assert ns >= 1, (n, t)
             assert np.abs(out - ref).max() / np.abs(ref).max() < 2e-6, (fn, n, t, d)
+
+
+@pytest.mark.parametrize("n,t,pad", [(32, 4, 8), (48, 4, 4), (64, 8, 8), (80, 4, 8), (96, 4, 16),
+                                     (96, 8, 16)])
+def test_group_fft_padded_exchanges_match_numpy(host, n, t, pad):
+    """The row FFTs' padded LDS exchanges (element p at p + p // pad) give the
+    same transform: in registers, and in natural order in the exchange buffer
+    for the form that leaves its result there; a dropped or misplaced slot
+    would scramble the output."""
+    f = host.pf_host_group_fft_pad
+    rng = np.random.default_rng(n * 100 + t)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    for d in (-1, 1):
+        ref = np.fft.fft(x.astype(np.complex128)) if d < 0 else np.fft.ifft(x) * n
+        for to_ex in (0, 1):
+            out = np.zeros(n, np.complex64)
+            slots = f(n, t, pad, to_ex, d, x.ctypes.data_as(FP), out.ctypes.data_as(FP))
+            assert slots == n + (n - 1) // pad
+            assert np.abs(out - ref).max() / np.abs(ref).max() < 1e-6
