@@ -401,6 +401,18 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
     }
 }
 
+// 1 / sqrt of the local variance from the raw (unnormalised) inverse-FFT
+// outputs A = V sum W T and B = V sum W^2 T^2 of one voxel: with nv = N V,
+// N V B - A^2 = V^2 (N <W^2 T^2> - <W T>^2).  B nv (24 x 45 bits) and A^2
+// (48 bits) are exact in FP64, so the difference has a single rounding; the
+// LCC is then G_raw * rsq (the factors V cancel).  Raw v_rsq_f32: the
+// argument is 0 (-> +inf), negative (-> NaN) or a normal float, never a
+// denormal (FP32 FFT outputs are 0 or far above 1e-38).
+__device__ __forceinline__ float rsq_var(float a, float b, double nv) {
+    const double ad = (double)a;
+    return __builtin_amdgcn_rsqf((float)fma(-ad, ad, (double)b * nv));
+}
+
 // ---------------------------------------------------------------------------
 // XPF (one-sweep column plans only): the next plane is loaded while the current
 // one is transformed (costs EY registers through the row phase); otherwise a
@@ -421,7 +433,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
                                                     const uint8_t* __restrict__ lmask,
                                                     unsigned long long* __restrict__ keys,
                                                     int nrot, int chunk, int64_t first,
-                                                    double norm, double invV,
+                                                    double norm,
                                                     const float2* __restrict__ twM,
                                                     const float2* __restrict__ twN) {
     using G = Geo<N>;
@@ -457,7 +469,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
     // column pass: lanes run across columns (coalesced loads, conflict-free LDS)
     const int cg0 = WCOLS ? threadIdx.x / TY : threadIdx.x % CPI;
     const int ty0 = WCOLS ? threadIdx.x % TY : threadIdx.x / CPI;
-    const float invVf = (float)invV;
+    const double nv = norm * (double)G::V;   // N V (exact)
 
     // owned voxels: (row = it*RPI + gx0, x = 2 (tx0 + TX e) + h), bit 2(it*EX+e)+h
     using MaskT = typename std::conditional<(ITERS * EX * 2 <= 32), uint32_t, uint64_t>::type;
@@ -573,16 +585,15 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
                 for (int e = 0; e < EX; ++e) {
                     float2 held = g == 2 ? vr[e] : slot[TX * e];
                     if (g == 1) {
-                        // N<W^2 T^2> - <W T>^2 in FP64, then 1/sqrt in FP32
-                        const double ax = (double)vr[e].x * invV, ay = (double)vr[e].y * invV;
-                        held.x = rsqrtf((float)((double)held.x * invV * norm - ax * ax));
-                        held.y = rsqrtf((float)((double)held.y * invV * norm - ay * ay));
+                        // N V B - A^2 in FP64 (exact products), then 1/sqrt in FP32
+                        held.x = rsq_var(vr[e].x, held.x, nv);
+                        held.y = rsq_var(vr[e].y, held.y, nv);
                     } else if (g == 0) {
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
                             if (!((mbits >> (2 * (it * EX + e) + h)) & MaskT(1))) continue;
                             // only S > 0 can beat the initial (0, 0); NaN never wins
-                            const float sv = (h ? vr[e].y : vr[e].x) * invVf * (h ? held.y : held.x);
+                            const float sv = (h ? vr[e].y : vr[e].x) * (h ? held.y : held.x);
                             if (sv > 0.f) {
                                 const int64_t vox =
                                     ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
@@ -602,13 +613,11 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
                         if (g == 2) {
                             is = val;
                         } else if (g == 1) {
-                            // N<W^2 T^2> - <W T>^2 in FP64, then 1/sqrt in FP32
-                            const double a = (double)val * invV;
-                            const double a2 = (double)is * invV * norm;
-                            is = rsqrtf((float)(a2 - a * a));
+                            // N V B - A^2 in FP64 (exact products), then 1/sqrt in FP32
+                            is = rsq_var(val, is, nv);
                         } else if ((mbits >> (2 * (it * EX + e) + h)) & MaskT(1)) {
                             // only S > 0 can beat the initial (0, 0); NaN never wins
-                            const float sv = val * invVf * is;
+                            const float sv = val * is;
                             if (sv > 0.f) {
                                 const int64_t vox =
                                     ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
@@ -758,47 +767,66 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane2(const float2* __restrict__ 
 //
 // The lane -> (row, float) pattern repeats every GR rows (GR * RF floats are a
 // whole number CG of 64-float chunks), so each lane computes its CG byte
-// offsets within such a group once per call; the copies go through a buffer
-// resource over the source plane, with the group's offset as the scalar
-// offset, so issuing them takes no vector arithmetic (and a copy outside the
-// plane's bytes would read zeros instead of faulting).
+// offsets within such a group once per kernel (PlaneCopy holds them); the
+// copies go through a buffer resource over the source plane, with the group's
+// offset as the scalar offset, so issuing them takes no vector arithmetic (and
+// a copy outside the plane's bytes would read zeros instead of faulting).
 constexpr int dma_gcd(int a, int b) { return b == 0 ? a : dma_gcd(b, a % b); }
 
-template <int N, int NTHR>
-__device__ __forceinline__ void dma_plane(float2* slot, const float2* __restrict__ src) {
+// HOIST: the offsets are computed once, when the PlaneCopy is made, and held
+// in CG registers; otherwise per call, behind an empty asm so the compiler
+// does not hoist them out of the caller's loop (for kernels short of
+// registers).
+template <int N, int NTHR, bool HOIST>
+struct PlaneCopy {
     using G = Geo<N>;
-    constexpr int RF = 2 * G::S;     // floats per LDS row
-    constexpr int GF = 2 * G::NXH;   // floats per global row
-    constexpr int GR = 64 / dma_gcd(RF, 64);   // rows per group
-    constexpr int CG = GR * RF / 64;            // chunks per group
-    constexpr int NG = N / GR;                  // groups per plane
-    constexpr int NW = NTHR / 64;               // waves
+    static constexpr int RF = 2 * G::S;     // floats per LDS row
+    static constexpr int GF = 2 * G::NXH;   // floats per global row
+    static constexpr int GR = 64 / dma_gcd(RF, 64);   // rows per group
+    static constexpr int CG = GR * RF / 64;            // chunks per group
+    static constexpr int NG = N / GR;                  // groups per plane
+    static constexpr int NW = NTHR / 64;               // waves
     static_assert(N % GR == 0 && NTHR % 64 == 0, "whole groups of rows, whole waves");
-    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float2*>(src), 0, (int)(sizeof(float) * N * GF), 0x00020000);
-    float* d = reinterpret_cast<float*>(slot);
-    // through an empty asm, so the offsets are computed per call instead of
-    // being hoisted out of the caller's loop (where they would hold CG
-    // registers across it)
-    int lane = threadIdx.x & 63;
-    asm volatile("" : "+v"(lane));
-    int off[CG];
+    int off[HOIST ? CG : 1];
+
+    // this lane's byte offset of chunk j within a group of rows
+    static __device__ __forceinline__ void offsets(int lane, int (&o)[CG]) {
 #pragma unroll
-    for (int j = 0; j < CG; ++j) {
-        const int f = j * 64 + lane;
-        const int row = f / RF, fc = f - row * RF;
-        off[j] = 4 * (row * GF + (fc < GF ? fc : 0));
+        for (int j = 0; j < CG; ++j) {
+            const int f = j * 64 + lane;
+            const int row = f / RF, fc = f - row * RF;
+            o[j] = 4 * (row * GF + (fc < GF ? fc : 0));
+        }
     }
-    // the wave index, as a wave-uniform (scalar) value
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    for (int g = wave; g < NG; g += NW) {
+
+    __device__ __forceinline__ PlaneCopy() {
+        if constexpr (HOIST) offsets(threadIdx.x & 63, off);
+    }
+
+    __device__ __forceinline__ void operator()(float2* slot, const float2* __restrict__ src) const {
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float2*>(src), 0, (int)(sizeof(float) * N * GF), 0x00020000);
+        float* d = reinterpret_cast<float*>(slot);
+        int o[CG];
+        if constexpr (HOIST) {
 #pragma unroll
-        for (int j = 0; j < CG; ++j)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                rsrc, (__attribute__((address_space(3))) void*)(d + g * (GR * RF) + j * 64), 4,
-                off[j], 4 * g * (GR * GF), 0, 0);
+            for (int j = 0; j < CG; ++j) o[j] = off[j];
+        } else {
+            int lane = threadIdx.x & 63;
+            asm volatile("" : "+v"(lane));
+            offsets(lane, o);
+        }
+        // the wave index, as a wave-uniform (scalar) value
+        const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+        for (int g = wave; g < NG; g += NW) {
+#pragma unroll
+            for (int j = 0; j < CG; ++j)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rsrc, (__attribute__((address_space(3))) void*)(d + g * (GR * RF) + j * 64),
+                    4, o[j], 4 * g * (GR * GF), 0, 0);
+        }
     }
-}
+};
 
 // Wait for this wave's outstanding global loads (LDS copies included) and
 // stores, then for the whole workgroup.
@@ -813,12 +841,12 @@ __device__ __forceinline__ void wait_copies_and_sync() {
 // transformed from slot 0 into slot 1.  Planes reach LDS by direct copies
 // issued while the other slot is being transformed: G during the A rows, the
 // next B during G's transform; only the next A's copy is waited for.
-template <int N, int NTHR, int EMAX>
+template <int N, int NTHR, int EMAX, bool HOISTC = false>
 __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ X,
                                                      const uint8_t* __restrict__ lmask,
                                                      unsigned long long* __restrict__ keys,
                                                      int nrot, int chunk, int64_t first,
-                                                     double norm, double invV,
+                                                     double norm,
                                                      const float2* __restrict__ twM,
                                                      const float2* __restrict__ twN) {
     using G = Geo<N>;
@@ -849,7 +877,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
     const bool cactp = csp < 2;
     const int ccs = threadIdx.x % CPIS, tys = threadIdx.x / CPIS;
     const bool cacts = ccs < M;
-    const float invVf = (float)invV;
+    const double nv = norm * (double)G::V;   // N V (exact)
 
     // owned voxels: (row, x = 2 (tx + TX e) + h), bit 2 e + h
     uint32_t mbits = 0;
@@ -887,9 +915,10 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
         group_fft<M, TX, 1, WROWS, XPAD>(vr, rowp, 1, tx, stwM, ract);
     };
 
+    const PlaneCopy<N, NTHR, HOISTC> copy;
     if (b0 < b1) {
-        dma_plane<N, NTHR>(slot0, plane_of(b0, 2));   // B
-        dma_plane<N, NTHR>(slot1, plane_of(b0, 1));   // A
+        copy(slot0, plane_of(b0, 2));   // B
+        copy(slot1, plane_of(b0, 1));   // A
     }
     for (int b = b0; b < b1; ++b) {
         wait_copies_and_sync();   // B and A of rotation b are in the slots
@@ -921,7 +950,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
             }
         }
         __syncthreads();   // slot 0 is free
-        dma_plane<N, NTHR>(slot0, plane_of(b, 0));   // G
+        copy(slot0, plane_of(b, 0));   // G
         {
             float2 vr[EX];
             row_inverse(slot1 + (ract ? row : 0) * S, vr);
@@ -930,10 +959,8 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
                 for (int e = 0; e < EX; ++e)
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
-                        // N<W^2 T^2> - <W T>^2 in FP64, then 1/sqrt in FP32
-                        const double av = (double)(h ? vr[e].y : vr[e].x) * invV;
-                        const double a2 = (double)isd[e][h] * invV * norm;
-                        isd[e][h] = rsqrtf((float)(a2 - av * av));
+                        // N V B - A^2 in FP64 (exact products), then 1/sqrt in FP32
+                        isd[e][h] = rsq_var(h ? vr[e].y : vr[e].x, isd[e][h], nv);
                     }
             }
         }
@@ -949,7 +976,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
                 }
             }
             __syncthreads();   // slot 0 is read: the next B can come in
-            if (b + 1 < b1) dma_plane<N, NTHR>(slot0, plane_of(b + 1, 2));
+            if (b + 1 < b1) copy(slot0, plane_of(b + 1, 2));
             group_fft_ex<N, TYS, 1, false>(v, slot1 + (cacts ? ccs : 0), S, tys, stwN, cacts);
         }
         __syncthreads();
@@ -963,7 +990,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
                     for (int h = 0; h < 2; ++h) {
                         if (!((mbits >> (2 * e + h)) & 1u)) continue;
                         // only S > 0 can beat the initial (0, 0); NaN never wins
-                        const float sv = (h ? vr[e].y : vr[e].x) * invVf * isd[e][h];
+                        const float sv = (h ? vr[e].y : vr[e].x) * isd[e][h];
                         if (sv > 0.f) {
                             const int64_t vox =
                                 ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
@@ -973,7 +1000,7 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
             }
         }
         __syncthreads();   // slot 1 is free
-        if (b + 1 < b1) dma_plane<N, NTHR>(slot1, plane_of(b + 1, 1));
+        if (b + 1 < b1) copy(slot1, plane_of(b + 1, 1));
     }
 }
 
@@ -1088,6 +1115,7 @@ struct Shape {
                          EMAXZ2 = EZ2_, EMAX3 = E_, EMAXC3 = E_, EMAXC1 = E_;
     static constexpr int TQ2 = TQ_ == 32 ? 16 : 32;   // alternative z tile width
     static constexpr bool XPF3 = true, ISDG3 = false, RECOMP3 = false, STASHW = false;
+    static constexpr bool HOIST3 = false;   // two-plane K3: copy offsets held in registers
     static constexpr bool PAIR1 = true, PAIR3 = true;   // two-plane K1 / K3 where they fit
 };
 
@@ -1102,7 +1130,7 @@ struct Shape80 : Shape<320, 16, 16, 320, 12, 24> {
 // 128: one-plane K1 in 512-thread blocks, one launch per grid (7.3 against
 // 8.4 us per rotation for the two-plane K1)
 struct Shape128 : Shape<512, 32, 16, 1024, 16, 8> {
-    static constexpr bool PAIR1 = false, STASHW = true;
+    static constexpr bool PAIR1 = false, STASHW = true, HOIST3 = true;
 };
 struct Shape144 : Shape<864, 16, 12, 864, 16, 6> {
     static constexpr bool RECOMP3 = true;
@@ -1162,15 +1190,15 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
         const int ch = a.chunk_lcc;
         // two-slot inverse only for the large planes: it costs registers
         if constexpr (SH::PAIR3 && PairPlan<N, NTHR3, EMAX>::ok && NTHR3 >= 512)
-            hipLaunchKernelGGL((k_inv_plane2<N, NTHR3, EMAX>), dim3(N, (a.nrot + ch - 1) / ch),
+            hipLaunchKernelGGL((k_inv_plane2<N, NTHR3, EMAX, SH::HOIST3>), dim3(N, (a.nrot + ch - 1) / ch),
                                dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
-                               a.norm, a.invV, tw.M, tw.N);
+                               a.norm, tw.M, tw.N);
         else
             hipLaunchKernelGGL((k_inv_plane<N, NTHR3, SH::EMAX3, SH::EMAXC3, SH::XPF3, SH::ISDG3,
                                             SH::RECOMP3>),
                                dim3(N, (a.nrot + ch - 1) / ch),
                                dim3(NTHR3), 0, s, a.work, a.lcc_mask, a.keys, a.nrot, ch, a.first,
-                               a.norm, a.invV, tw.M, tw.N);
+                               a.norm, tw.M, tw.N);
         PF_CHECK_STATUS(last("k_inv_plane"));
     }
     return kOk;
