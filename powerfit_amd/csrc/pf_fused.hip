@@ -218,6 +218,9 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
     const int ty = WCOLS ? threadIdx.x % TY : threadIdx.x / CPI;
     const bool cact = cg < M;
     const int sz = signed_coord<N>(iz, r);
+    // gathered voxels: inside the rotation sphere and the support ball (|v| >
+    // zr reads only zeros whatever the rotation)
+    const int rg = r < zr ? r : zr, rg2 = rg * rg;
     double R[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                             const int x = signed_coord<N>(2 * m + h, r);
                             pv[h] = 0.f;
                             wv[h] = 0.f;
-                            if (sz * sz + sy * sy + x * x <= r * r)
+                            if (sz * sz + sy * sy + x * x <= rg2)
                                 gather_pw<N>(tm, occ, R, sz, sy, x, &pv[h], &wv[h]);
                         }
                         v[e] = c2(pv[0], pv[1]);
@@ -280,7 +283,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
                         if (g == 0) {
                             const int x = signed_coord<N>(2 * m + h, r);
                             float p = 0.f, wv = 0.f;
-                            if (sz * sz + sy * sy + x * x <= r * r)
+                            if (sz * sz + sy * sy + x * x <= rg2)
                                 gather_pw<N>(tm, occ, R, sz, sy, x, &p, &wv);
                             pv[h] = p;
                             w[WSG ? 0 : it][WSG ? 0 : e][h] = wv;
@@ -685,6 +688,9 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane2(const float2* __restrict__ 
     const int iz = (blockIdx.x + zlo + N) % N;
     const int b = blockIdx.y;
     const int sz = signed_coord<N>(iz, r);
+    // gathered voxels: inside the rotation sphere and the support ball (|v| >
+    // zr reads only zeros whatever the rotation)
+    const int rg = r < zr ? r : zr, rg2 = rg * rg;
     double R[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
@@ -714,7 +720,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane2(const float2* __restrict__ 
                     const int x = signed_coord<N>(2 * (tx + TX * e) + h, r);
                     p2[h] = 0.f;
                     w2[h] = 0.f;
-                    if (sz * sz + sy * sy + x * x <= r * r)
+                    if (sz * sz + sy * sy + x * x <= rg2)
                         gather_pw<N>(tm, occ, R, sz, sy, x, &p2[h], &w2[h]);
                 }
                 vp[e] = c2(p2[0], p2[1]);
