@@ -253,7 +253,7 @@ def main(argv=None):
                          "(with --batch; default automatic)")
     ap.add_argument("--z-schedule", type=int, default=0,
                     help="schedule of the fused z stage (with --batch)")
-    ap.add_argument("--mode", default="fused", choices=["fused", "batched", "plain", "auto"])
+    ap.add_argument("--mode", default="fused", choices=["fused", "fused_rt", "batched", "plain", "auto"])
     ap.add_argument("--workload", default="search128")
     ap.add_argument("--plain-rotations", type=int, default=256,
                     help="rotations timed on the plain one-rotation-per-launch path")

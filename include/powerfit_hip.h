@@ -41,6 +41,11 @@ extern "C" {
 #define PF_MODE_MIX_FWD 4  /* fused rotate + forward xy                             */
 #define PF_MODE_MIX_Z 5    /* fused z correlate                                     */
 #define PF_MODE_MIX_INV 6  /* fused inverse xy + LCC                                */
+/* The three fused stages with run-time FFT plans (LDS Stockham passes): grids
+ * without a compiled fused size (pf_ctx_fused_available returns 2): even nx,
+ * an ny x (nx/2+1) complex plane twice in LDS, ny nx <= 16384.  PF_MODE_AUTO
+ * takes it where the compiled pipeline does not exist. */
+#define PF_MODE_FUSED_RT 7
 
 /* pf_ctx_create flags */
 #define PF_FLAG_NO_BINARY_MASK 1u /* never use the W^2 = W shortcut for 0/1 masks  */
@@ -184,7 +189,8 @@ int pf_ctx_reached(pf_ctx *ctx, uint64_t ticket);
 #define PF_K_ZCORR 8
 #define PF_K_INV_PLANE 9
 #define PF_K_COUNT 10
-/* 1 when PF_MODE_FUSED is available for this ctx's grid. */
+/* 1 when PF_MODE_FUSED (compiled kernels) is available for this ctx's grid,
+ * 2 when only PF_MODE_FUSED_RT (run-time plans) is, 0 when neither is. */
 int pf_ctx_fused_available(pf_ctx *ctx);
 /* Rotations per block in the fused z-correlation and LCC kernels (0 = choose
  * from the batch size so that each launch has at least 4 blocks per CU).

@@ -13,7 +13,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "powerfit_hip.h")
 
 ABI_VERSION = 2
 MODES = {"plain": 0, "batched": 1, "fused": 2, "auto": 3,
-         "mix_fwd": 4, "mix_z": 5, "mix_inv": 6}
+         "mix_fwd": 4, "mix_z": 5, "mix_inv": 6, "fused_rt": 7}
 MODE_PLAIN = MODES["plain"]
 MODE_BATCHED = MODES["batched"]
 FLAG_NO_BINARY_MASK = 1

@@ -109,7 +109,9 @@ class HipCorrelator(BaseCorrelator):
     mode  : "plain" (one rotation per launch chain, unfused kernels),
             "batched" (``batch`` rotations per launch, unfused kernels),
             "fused" (``batch`` rotations per launch through the three fused
-            kernels; cubic grids of a compiled size), or "auto" (fused where
+            kernels; cubic grids of a compiled size), "fused_rt" (the three
+            fused stages with run-time FFT plans: other grids with even nx
+            whose planes fit LDS), or "auto" (fused, else fused_rt where
             available, else batched).  "mix_fwd"/"mix_z"/"mix_inv" run one
             fused stage with the rest unfused (for testing).
     batch : rotations per launch (256 measured best at every fused size; a
@@ -230,7 +232,13 @@ class HipCorrelator(BaseCorrelator):
 
     @property
     def fused_available(self):
-        return bool(self._libref.pf_ctx_fused_available(self._ctx))
+        """The compiled fused pipeline exists for this grid."""
+        return self._libref.pf_ctx_fused_available(self._ctx) == 1
+
+    @property
+    def fused_rt_available(self):
+        """Only the run-time planned fused pipeline exists for this grid."""
+        return self._libref.pf_ctx_fused_available(self._ctx) == 2
 
     def enqueue_scan_graph(self, first, count):
         """Enqueue [first, first+count) as one replay of a captured HIP graph."""

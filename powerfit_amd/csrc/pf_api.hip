@@ -192,11 +192,19 @@ int ensure_capacity(pf_ctx* c, int batch, bool need_real) {
     return kOk;
 }
 
+// The run-time planned fused stages: their grid limits, and a rotation
+// radius within every axis's half length (signed coordinates per axis).
+bool rt_available(const pf_ctx* c) {
+    const int half = std::min(c->nz, std::min(c->ny, c->nx)) / 2;
+    return plane_rt_supported(c->nz, c->ny, c->nx) && c->rmax <= half;
+}
+
 // Scan request after validation: mode resolved (AUTO), batch clamped to the
 // range, and which of the three stages run fused.
 struct ScanPlan {
     int batch = 0, mode = 0;
     bool fa = false, fb = false, fc = false;   // fused forward / z / inverse stage
+    bool rt = false;                            // the fused stages with run-time plans
     bool need_real = false;                     // real work grids (an unfused stage)
 };
 
@@ -208,13 +216,16 @@ int plan_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode, Scan
     }
     if (first < 0 || count < 0 || first + count > c->nrot)
         return arg_error("rotation range outside the rotation set");
-    if (mode < PF_MODE_PLAIN || mode > PF_MODE_MIX_INV) return arg_error("unknown mode");
+    if (mode < PF_MODE_PLAIN || mode > PF_MODE_FUSED_RT) return arg_error("unknown mode");
     // the split-precision FFT stage exists in the unfused passes only
     const bool split = (c->flags & PF_FLAG_SPLIT_DFT) != 0;
     const bool fused_ok =
         !split && fused_supported(c->nz, c->ny, c->nx) && c->rmax <= c->nx / 2;
-    if (mode == PF_MODE_AUTO) mode = fused_ok ? PF_MODE_FUSED : PF_MODE_BATCHED;
-    if (mode >= PF_MODE_FUSED && !fused_ok) {
+    const bool rt_ok = !split && rt_available(c);
+    if (mode == PF_MODE_AUTO)
+        mode = fused_ok ? PF_MODE_FUSED : (rt_ok ? PF_MODE_FUSED_RT : PF_MODE_BATCHED);
+    if ((mode == PF_MODE_FUSED_RT && !rt_ok) ||
+        (mode >= PF_MODE_FUSED && mode != PF_MODE_FUSED_RT && !fused_ok)) {
         set_error(split ? "fused pipeline not available with PF_FLAG_SPLIT_DFT"
                         : "fused pipeline not available for grid (%d, %d, %d)",
                   c->nz, c->ny, c->nx);
@@ -228,9 +239,10 @@ int plan_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode, Scan
     sp->batch = batch;
     sp->mode = mode;
     // stage A: rotate + forward xy; B: z correlate; C: inverse xy + LCC
-    sp->fa = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_FWD;
-    sp->fb = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_Z;
-    sp->fc = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_INV;
+    sp->rt = mode == PF_MODE_FUSED_RT;
+    sp->fa = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_FWD || sp->rt;
+    sp->fb = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_Z || sp->rt;
+    sp->fc = mode == PF_MODE_FUSED || mode == PF_MODE_MIX_INV || sp->rt;
     sp->need_real = !(sp->fa && sp->fc);
     return kOk;
 }
@@ -245,6 +257,11 @@ int enqueue_scan(pf_ctx* c, int64_t first, int64_t count, const ScanPlan& sp, fl
                  float* real, bool timing) {
     const int batch = sp.batch;
     const bool fa = sp.fa, fb = sp.fb, fc = sp.fc;
+    // one fused stage: the compiled kernels, or their run-time planned form
+    auto fused = [&](const FusedArgs& a, int which) {
+        return sp.rt ? launch_planes_rt(c->fft, a, which, c->stream)
+                     : launch_fused(a, which, c->stream);
+    };
     auto run = [&](int which, auto&& f) -> int { return timing ? timed(c, which, f) : f(); };
     const double invV = 1.0 / (double)c->V;
     const int64_t plane = (int64_t)c->ny * c->fft.nxh;
@@ -255,7 +272,7 @@ int enqueue_scan(pf_ctx* c, int64_t first, int64_t count, const ScanPlan& sp, fl
     fa_args.rmax = c->rmax;
     fa_args.zr = c->support_zr;
     fa_args.zsched = c->zsched;
-    fa_args.skip_planes = fa && fb;
+    fa_args.skip_planes = fa && fb && !sp.rt;
     fa_args.work = cplx;
     fa_args.F1 = c->d_F1;
     fa_args.F2 = c->d_F2;
@@ -280,7 +297,7 @@ int enqueue_scan(pf_ctx* c, int64_t first, int64_t count, const ScanPlan& sp, fl
         if (fa) {
             // W^2 = W shortcut only when the z stage is fused (it fills grid 2)
             fa_args.binary_mask = c->mask_binary && fb;
-            PF_CHECK_STATUS(run(PF_K_FWD_PLANE, [&] { return launch_fused(fa_args, 0, c->stream); }));
+            PF_CHECK_STATUS(run(PF_K_FWD_PLANE, [&] { return fused(fa_args, 0); }));
         } else {
             PF_CHECK_STATUS(run(PF_K_ROTATE, [&] {
                 return launch_rotate_pw(c->d_template, c->d_mask, c->d_rot + 9 * k0, nb, c->nz,
@@ -297,7 +314,7 @@ int enqueue_scan(pf_ctx* c, int64_t first, int64_t count, const ScanPlan& sp, fl
         }
         if (fb) {
             fa_args.binary_mask = c->mask_binary;
-            PF_CHECK_STATUS(run(PF_K_ZCORR, [&] { return launch_fused(fa_args, 1, c->stream); }));
+            PF_CHECK_STATUS(run(PF_K_ZCORR, [&] { return fused(fa_args, 1); }));
         } else {
             PF_CHECK_STATUS(run(PF_K_C2C_FWD, [&] {
                 return launch_axis(c->fft, 2, cplx, plane, plane, (int64_t)3 * nb,
@@ -312,7 +329,7 @@ int enqueue_scan(pf_ctx* c, int64_t first, int64_t count, const ScanPlan& sp, fl
             }));
         }
         if (fc) {
-            PF_CHECK_STATUS(run(PF_K_INV_PLANE, [&] { return launch_fused(fa_args, 2, c->stream); }));
+            PF_CHECK_STATUS(run(PF_K_INV_PLANE, [&] { return fused(fa_args, 2); }));
         } else {
             PF_CHECK_STATUS(run(PF_K_C2C_INV, [&] {
                 return launch_axis(c->fft, 1, cplx, c->fft.nxh, c->fft.nxh,
@@ -697,8 +714,11 @@ int64_t pf_ctx_bytes(int nz, int ny, int nx, int batch, int mode, uint32_t flags
     if (nrot > 0 && batch > nrot) batch = (int)nrot;   // pf_scan never uses more
     // the fused work layout holds only the complex spectra; every other mode,
     // and the split-precision FFT, also needs the real grids
-    const bool fused = (mode == PF_MODE_FUSED || mode == PF_MODE_AUTO) &&
-                       !(flags & PF_FLAG_SPLIT_DFT) && fused_supported(nz, ny, nx);
+    const bool compiled = (mode == PF_MODE_FUSED || mode == PF_MODE_AUTO) &&
+                          fused_supported(nz, ny, nx);
+    const bool rt = (mode == PF_MODE_FUSED_RT || mode == PF_MODE_AUTO) &&
+                    plane_rt_supported(nz, ny, nx);
+    const bool fused = !(flags & PF_FLAG_SPLIT_DFT) && (compiled || rt);
     // template, mask (f32), packed pair, keys, lcc mask, two target spectra,
     // brick occupancy, rotations
     int64_t bytes = V * (4 + 4 + 8 + 8 + 1) + 2 * H * 8 + nrot * 9 * 8;
@@ -816,10 +836,9 @@ int pf_scan_graph(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) 
 
 int pf_ctx_fused_available(pf_ctx* c) {
     if (!c) return arg_error("ctx is null");
-    return (!(c->flags & PF_FLAG_SPLIT_DFT) && fused_supported(c->nz, c->ny, c->nx) &&
-            (c->rmax < 0 || c->rmax <= c->nx / 2))
-               ? 1
-               : 0;
+    if (c->flags & PF_FLAG_SPLIT_DFT) return 0;
+    if (fused_supported(c->nz, c->ny, c->nx) && (c->rmax < 0 || c->rmax <= c->nx / 2)) return 1;
+    return rt_available(c) ? 2 : 0;
 }
 
 int pf_get_result(pf_ctx* c, float* best, int32_t* idx) {

@@ -191,4 +191,38 @@ PF_HD void stage_dispatch(const float2* sp, float2* dp, int R, int j, int Ns, in
     }
 }
 
+#if defined(__HIP__)
+// LDS FFT driver: run all stages of plan p over `nfft` transforms held in LDS
+// (element e of transform f at a[f * fs + e * es]), ping-ponging between a
+// and b.  Returns the buffer holding the result.  Ends with a barrier.
+template <int DIR>
+__device__ float2* lds_fft(float2* a, float2* b, const Plan1D& p, const float2* tw, int nfft,
+                           int fs, int es) {
+    int Ns = 1;
+    for (int s = 0; s < p.nstages; ++s) {
+        const int R = p.radix[s];
+        const int nb = p.n / R;
+        const int total = nfft * nb;
+        for (int u = threadIdx.x; u < total; u += blockDim.x) {
+            int f, j;
+            if (es == 1) {
+                f = u / nb;
+                j = u - f * nb;
+            } else {
+                j = u / nfft;
+                f = u - j * nfft;
+            }
+            stage_dispatch<DIR>(a + f * fs, b + f * fs, R, j, Ns, p.n, es, tw);
+        }
+        __syncthreads();
+        float2* t = a;
+        a = b;
+        b = t;
+        Ns *= R;
+    }
+    return a;
+}
+
+#endif
+
 }  // namespace pf

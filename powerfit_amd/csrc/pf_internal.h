@@ -130,6 +130,13 @@ struct FusedArgs {
 // forward xy, 1: z correlate, 2: inverse xy + LCC) or -1 for all three.
 int launch_fused(const FusedArgs& a, int which, hipStream_t s);
 int launch_pack_tm(const float* tmpl, const float* mask, float2* tm, int64_t n, hipStream_t s);
+// ---- runtime-planned fused path (pf_planes.hip): any grid within its limits ----
+bool plane_rt_supported(int nz, int ny, int nx);
+// Stage `which` (0, 1, 2 as launch_fused) of the runtime-planned pipeline; uses
+// f's plans and twiddles and a's buffers and options (a.n, a.occ and
+// a.skip_planes are not used: every plane is transformed).
+int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t s);
+
 // 1 bit per 8^3 brick of an n^3 interleaved grid (n % 8 == 0): ((n/8)^3 + 31) / 32 words.
 int launch_brick_occupancy(const float2* tm, int n, uint32_t* occ, hipStream_t s);
 

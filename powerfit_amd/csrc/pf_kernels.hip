@@ -17,38 +17,6 @@
 namespace pf {
 
 // ---------------------------------------------------------------------------
-// LDS FFT driver: run all stages of plan p over `nfft` transforms held in LDS
-// (element e of transform f at a[f * fs + e * es]), ping-ponging between a
-// and b.  Returns the buffer holding the result.  Ends with a barrier.
-template <int DIR>
-__device__ float2* lds_fft(float2* a, float2* b, const Plan1D& p, const float2* tw, int nfft,
-                           int fs, int es) {
-    int Ns = 1;
-    for (int s = 0; s < p.nstages; ++s) {
-        const int R = p.radix[s];
-        const int nb = p.n / R;
-        const int total = nfft * nb;
-        for (int u = threadIdx.x; u < total; u += blockDim.x) {
-            int f, j;
-            if (es == 1) {
-                f = u / nb;
-                j = u - f * nb;
-            } else {
-                j = u / nfft;
-                f = u - j * nfft;
-            }
-            stage_dispatch<DIR>(a + f * fs, b + f * fs, R, j, Ns, p.n, es, tw);
-        }
-        __syncthreads();
-        float2* t = a;
-        a = b;
-        b = t;
-        Ns *= R;
-    }
-    return a;
-}
-
-// ---------------------------------------------------------------------------
 // R2C along the contiguous x axis: rows of nx reals -> nx/2+1 complex.
 __global__ void k_r2c_x(const float* __restrict__ in, float2* __restrict__ out, int nx,
                         int64_t rows, Plan1D p, const float2* __restrict__ tw,

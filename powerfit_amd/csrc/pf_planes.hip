@@ -1,0 +1,382 @@
+// Runtime-planned fused LCC pipeline for grids without a compiled fused size
+// (non-cubic grids, and cubic lengths whose radices the register FFTs cannot
+// split evenly over threads, e.g. 120 = 8 * 3 * 5).
+//
+// The same three stages as the compiled pipeline (pf_fused.hip), with the FFT
+// lengths and radix plans taken at run time: every transform is an LDS
+// Stockham pass (lds_fft) over a whole z-plane or a tile of z-columns.
+//
+//   k_fwd_plane_rt  one block per (z-plane, rotation): gathers the rotated
+//                   template (trilinear) and mask (nearest) of the plane row
+//                   by row, R2C along x (half-length complex FFT + post-
+//                   processing), C2C along y, and writes the xy-spectra of P,
+//                   W (and W^2 for weighted masks) in numpy's [z][ky][kx]
+//                   layout.
+//   k_zcorr_rt      one block per (tile of (ky,kx) columns, rotation chunk):
+//                   forward FFT along z, conj(X) F1 / conj(X) F2 with the F
+//                   tile in LDS, inverse FFT along z.
+//   k_inv_plane_rt  one block per (z-plane, rotation chunk): inverse C2C along
+//                   y and C2R along x for B, A then G, the LCC from the raw
+//                   outputs, merged into the running (LCC, index) keys with
+//                   atomicMax.
+//
+// Limits (plane_rt_supported): even nx; two planes of ny x (nx/2+1) complex
+// in LDS; at most kRtVox voxels of a plane per thread of the LCC stage.
+#include "pf_internal.h"
+#include "pf_rotate.h"
+
+namespace pf {
+
+namespace {
+
+constexpr int kRtThreads = 512;          // plane kernels
+constexpr int kRtVox = 32;               // voxels of a plane owned per thread (K3)
+constexpr int kRtZThreads = 256;         // z kernel
+constexpr int kRtTq = 16;                // z columns per block
+constexpr size_t kRtLds = 150 * 1024;    // dynamic LDS a plane kernel may take
+
+// Signed coordinate of index i on an axis of length n for rotation radius
+// r <= n / 2: i for i <= r, else i - n (as the compiled pipeline's).
+__device__ __forceinline__ int scoord(int i, int r, int n) { return i <= r ? i : i - n; }
+
+// Rotated template (trilinear, FP32 blend) and mask (nearest) at signed voxel
+// (z, y, x) from the interleaved (template, mask) grid, any shape.  FP64
+// coordinates, floor and round in the reference's order (so corner and
+// nearest indices are the reference's), one "+n" wrap for negatives, the
+// upper neighbour wrapping only at n.
+__device__ __forceinline__ void gather_pw_rt(const float2* __restrict__ tm, int nz, int ny,
+                                             int nx, const double* R, int z, int y, int x,
+                                             float* p, float* w) {
+    const Coord c = source_coord(R, z, y, x);
+    const double fx = floor(c.xs), fy = floor(c.ys), fz = floor(c.zs);
+    const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
+    const int xi = x0 + (x0 < 0 ? nx : 0), yi = y0 + (y0 < 0 ? ny : 0), zi = z0 + (z0 < 0 ? nz : 0);
+    *p = 0.f;
+    *w = 0.f;
+    if ((unsigned)xi >= (unsigned)nx || (unsigned)yi >= (unsigned)ny || (unsigned)zi >= (unsigned)nz)
+        return;
+    const int xj = (xi + 1 == nx) ? 0 : xi + 1;
+    const int yj = (yi + 1 == ny) ? 0 : yi + 1;
+    const int zj = (zi + 1 == nz) ? 0 : zi + 1;
+    const float2* r00 = tm + ((int64_t)zi * ny + yi) * nx;
+    const float2* r10 = tm + ((int64_t)zi * ny + yj) * nx;
+    const float2* r01 = tm + ((int64_t)zj * ny + yi) * nx;
+    const float2* r11 = tm + ((int64_t)zj * ny + yj) * nx;
+    const float2 a000 = r00[xi], a001 = r00[xj], a010 = r10[xi], a011 = r10[xj];
+    const float2 a100 = r01[xi], a101 = r01[xj], a110 = r11[xi], a111 = r11[xj];
+    const float dx = (float)(c.xs - fx), dy = (float)(c.ys - fy), dz = (float)(c.zs - fz);
+    const float c00 = a000.x + (a001.x - a000.x) * dx;
+    const float c10 = a010.x + (a011.x - a010.x) * dx;
+    const float c01 = a100.x + (a101.x - a100.x) * dx;
+    const float c11 = a110.x + (a111.x - a110.x) * dx;
+    const float c0 = c00 + (c10 - c00) * dy;
+    const float c1 = c01 + (c11 - c01) * dy;
+    *p = c0 + (c1 - c0) * dz;
+    // round() is half away from zero: up from floor when the fraction
+    // exceeds one half, or equals it for non-negative values
+    const double hx = c.xs - fx, hy = c.ys - fy, hz = c.zs - fz;
+    const bool ux = hx > 0.5 || (hx == 0.5 && c.xs >= 0.0);
+    const bool uy = hy > 0.5 || (hy == 0.5 && c.ys >= 0.0);
+    const bool uz = hz > 0.5 || (hz == 0.5 && c.zs >= 0.0);
+    const float2 n0 = uy ? (ux ? a011 : a010) : (ux ? a001 : a000);
+    const float2 n1 = uy ? (ux ? a111 : a110) : (ux ? a101 : a100);
+    *w = uz ? n1.y : n0.y;
+}
+
+struct RtGeo {
+    int nz, ny, nx, M, nxh, plane;   // M = nx / 2, nxh = M + 1, plane = ny * nxh
+    int64_t H;
+};
+
+// Forward stage.  A and B are two LDS planes of ny x nxh complex.
+__global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
+    const float2* __restrict__ tm, const double* __restrict__ rots, RtGeo g, int r, int zr,
+    int ngrid, float2* __restrict__ out, Plan1D px, Plan1D py, const float2* __restrict__ twx,
+    const float2* __restrict__ twy, const float2* __restrict__ twfull) {
+    extern __shared__ float2 lds[];
+    float2* const A = lds;
+    float2* const B = lds + g.plane;
+    const int iz = blockIdx.x, b = blockIdx.y;
+    const int sz = scoord(iz, r, g.nz);
+    // gathered voxels: inside the rotation sphere and the support ball
+    const int rg = r < zr ? r : zr, rg2 = rg * rg;
+    double R[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
+    float2* dst = out + (int64_t)b * 3 * g.H + (int64_t)iz * g.plane;
+    const int rowM = g.ny * g.M;
+    for (int gi = 0; gi < ngrid; ++gi) {
+        // rows: (v(2m), v(2m+1)) packed as complex, row stride M
+        for (int u = threadIdx.x; u < rowM; u += blockDim.x) {
+            const int y = u / g.M, m = u - y * g.M;
+            const int sy = scoord(y, r, g.ny);
+            float v2[2] = {0.f, 0.f};
+            if (sz * sz + sy * sy <= rg2) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int sx = scoord(2 * m + h, r, g.nx);
+                    if (sz * sz + sy * sy + sx * sx <= rg2) {
+                        float p, w;
+                        gather_pw_rt(tm, g.nz, g.ny, g.nx, R, sz, sy, sx, &p, &w);
+                        v2[h] = gi == 0 ? p : (gi == 1 ? w : w * w);
+                    }
+                }
+            }
+            A[u] = c2(v2[0], v2[1]);
+        }
+        __syncthreads();
+        float2* Z = lds_fft<-1>(A, B, px, twx, g.ny, g.M, 1);
+        float2* Xp = Z == A ? B : A;
+        // R2C post-processing, rows of nxh: X[k] = E[k] + w^k O[k]
+        for (int u = threadIdx.x; u < g.plane; u += blockDim.x) {
+            const int y = u / g.nxh, k = u - y * g.nxh;
+            const float2* z = Z + y * g.M;
+            float2 X;
+            if (k == 0 || k == g.M) {
+                const float2 z0 = z[0];
+                X = c2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.f);
+            } else {
+                const float2 zk = z[k], zm = cconj(z[g.M - k]);
+                const float2 E = cscale(cadd(zk, zm), 0.5f);
+                const float2 O = mul_si<-1>(cscale(csub(zk, zm), 0.5f));
+                X = cadd(E, cmul(twfull[k], O));
+            }
+            Xp[u] = X;
+        }
+        __syncthreads();
+        // y FFT of the nxh columns (element stride nxh)
+        const float2* Y = lds_fft<-1>(Xp, Z, py, twy, g.nxh, 1, g.nxh);
+        float2* o = dst + (int64_t)gi * g.H;
+        for (int u = threadIdx.x; u < g.plane; u += blockDim.x) o[u] = Y[u];
+        __syncthreads();
+    }
+}
+
+// z stage.  LDS: S0, S1, S2 (nz x TQ complex each, element (z, c) at z TQ + c)
+// and the F1, F2 tiles.  BIN: W's spectrum gives both A (F1) and B (F2).
+__global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
+    float2* __restrict__ X, const float2* __restrict__ F1, const float2* __restrict__ F2,
+    RtGeo g, int nrot, int chunk, int bin, Plan1D pz, const float2* __restrict__ twz) {
+    extern __shared__ float2 lds[];
+    const int tile = g.nz * kRtTq;
+    float2* const S0 = lds;
+    float2* const S1 = lds + tile;
+    float2* const S2 = lds + 2 * tile;
+    float2* const T1 = lds + 3 * tile;
+    float2* const T2 = lds + 4 * tile;
+    const int q0 = blockIdx.x * kRtTq;
+    const int nc = min(kRtTq, g.plane - q0);
+    const int b0 = blockIdx.y * chunk, b1 = min(nrot, b0 + chunk);
+    const int n = g.nz * nc;
+    for (int u = threadIdx.x; u < n; u += blockDim.x) {
+        const int z = u / nc, c = u - z * nc;
+        T1[z * kRtTq + c] = F1[(int64_t)z * g.plane + q0 + c];
+        T2[z * kRtTq + c] = F2[(int64_t)z * g.plane + q0 + c];
+    }
+    auto load = [&](const float2* src, float2* d) {
+        for (int u = threadIdx.x; u < n; u += blockDim.x) {
+            const int z = u / nc, c = u - z * nc;
+            d[z * kRtTq + c] = src[(int64_t)z * g.plane + q0 + c];
+        }
+        __syncthreads();
+    };
+    auto store = [&](const float2* s, float2* d) {
+        for (int u = threadIdx.x; u < n; u += blockDim.x) {
+            const int z = u / nc, c = u - z * nc;
+            d[(int64_t)z * g.plane + q0 + c] = s[z * kRtTq + c];
+        }
+    };
+    // in place: s = conj(s) * t (and, with t2, d = conj(s) * t2 first)
+    auto mul = [&](float2* s, const float2* t, float2* d, const float2* t2) {
+        for (int u = threadIdx.x; u < n; u += blockDim.x) {
+            const int z = u / nc, c = u - z * nc;
+            const int i = z * kRtTq + c;
+            const float2 x = cconj(s[i]);
+            if (d) d[i] = cmul(x, t2[i]);
+            s[i] = cmul(x, t[i]);
+        }
+        __syncthreads();
+    };
+    for (int b = b0; b < b1; ++b) {
+        float2* base = X + (int64_t)b * 3 * g.H;
+        // G = IFFT(conj(FP) F1)
+        load(base, S0);
+        float2* Z = lds_fft<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
+        mul(Z, T1, nullptr, nullptr);
+        const float2* Zi = lds_fft<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
+        store(Zi, base);
+        __syncthreads();
+        // A = IFFT(conj(FW) F1); B = IFFT(conj(FW) F2) or IFFT(conj(FW^2) F2)
+        load(base + g.H, S0);
+        Z = lds_fft<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
+        if (bin) {
+            mul(Z, T1, S2, T2);
+            Zi = lds_fft<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
+            store(Zi, base + g.H);
+            __syncthreads();
+            Zi = lds_fft<1>(S2, S0, pz, twz, nc, 1, kRtTq);
+            store(Zi, base + 2 * g.H);
+        } else {
+            mul(Z, T1, nullptr, nullptr);
+            Zi = lds_fft<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
+            store(Zi, base + g.H);
+            __syncthreads();
+            load(base + 2 * g.H, S0);
+            Z = lds_fft<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
+            mul(Z, T2, nullptr, nullptr);
+            Zi = lds_fft<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
+            store(Zi, base + 2 * g.H);
+        }
+        __syncthreads();
+    }
+}
+
+// Inverse stage and LCC.  Each thread owns voxels u = threadIdx.x + k * blockDim.x
+// (k < kRtVox) of the plane, row-major (y, x); it keeps B, then 1 / sqrt(N V B
+// - A^2), of them in registers.
+__global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
+    const float2* __restrict__ X, const uint8_t* __restrict__ lmask,
+    unsigned long long* __restrict__ keys, RtGeo g, int nrot, int chunk, int64_t first,
+    double norm, Plan1D px, Plan1D py, const float2* __restrict__ twx,
+    const float2* __restrict__ twy, const float2* __restrict__ twfull) {
+    extern __shared__ float2 lds[];
+    float2* const A = lds;
+    float2* const B = lds + g.plane;
+    const int iz = blockIdx.x;
+    const int b0 = blockIdx.y * chunk, b1 = min(nrot, b0 + chunk);
+    const int nvox = g.ny * g.nx;
+    const double nv = norm * ((double)g.nz * g.ny * g.nx);   // N V (exact)
+    const int64_t vz = (int64_t)iz * nvox;
+    uint32_t mbits = 0;
+#pragma unroll
+    for (int k = 0; k < kRtVox; ++k) {
+        const int u = threadIdx.x + k * blockDim.x;
+        if (u < nvox && lmask[vz + u]) mbits |= 1u << k;
+    }
+    float held[kRtVox];
+    for (int b = b0; b < b1; ++b) {
+        for (int gi = 2; gi >= 0; --gi) {
+            const float2* src = X + ((int64_t)b * 3 + gi) * g.H + (int64_t)iz * g.plane;
+            for (int u = threadIdx.x; u < g.plane; u += blockDim.x) A[u] = src[u];
+            __syncthreads();
+            // inverse y over the nxh columns
+            float2* Y = lds_fft<1>(A, B, py, twy, g.nxh, 1, g.nxh);
+            float2* D = Y == A ? B : A;
+            // C2R pre-processing, rows of M: z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
+            const int rowM = g.ny * g.M;
+            for (int u = threadIdx.x; u < rowM; u += blockDim.x) {
+                const int y = u / g.M, k = u - y * g.M;
+                const float2* xr = Y + y * g.nxh;
+                float2 xk = xr[k], xm = xr[g.M - k];
+                if (k == 0) {
+                    xk.y = 0.f;
+                    xm.y = 0.f;
+                }
+                const float2 xmc = cconj(xm);
+                const float2 d = cmul(csub(xk, xmc), cconj(twfull[k]));
+                D[u] = cadd(cadd(xk, xmc), mul_si<1>(d));
+            }
+            __syncthreads();
+            const float2* Z = lds_fft<1>(D, Y, px, twx, g.ny, g.M, 1);
+            const float* zr = reinterpret_cast<const float*>(Z);   // row y: 2 M reals
+#pragma unroll
+            for (int k = 0; k < kRtVox; ++k) {
+                const int u = threadIdx.x + k * blockDim.x;
+                if (u >= nvox) continue;
+                const float val = zr[u];   // (y, x) -> real 2 (y M) + x = u
+                if (gi == 2) {
+                    held[k] = val;
+                } else if (gi == 1) {
+                    const double ad = (double)val;
+                    held[k] = __builtin_amdgcn_rsqf((float)fma(-ad, ad, (double)held[k] * nv));
+                } else if ((mbits >> k) & 1u) {
+                    // only S > 0 can beat the initial (0, 0); NaN never wins
+                    const float sv = val * held[k];
+                    if (sv > 0.f) atomicMax(keys + vz + u, lcc_key(sv, (uint32_t)(first + b)));
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+int launched(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("launch of %s failed: %s", what, hipGetErrorString(e));
+        return kErrHip;
+    }
+    return kOk;
+}
+
+// Let a kernel take more than the default dynamic LDS (once per process and
+// device; the attribute is per function).
+int allow_lds(const void* fn, size_t bytes) {
+    const hipError_t e =
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize, %zu): %s", bytes,
+                  hipGetErrorString(e));
+        return kErrHip;
+    }
+    return kOk;
+}
+
+}  // namespace
+
+bool plane_rt_supported(int nz, int ny, int nx) {
+    if (nx % 2 != 0 || nz < 2 || ny < 2 || nx < 4) return false;
+    const int64_t plane = (int64_t)ny * (nx / 2 + 1);
+    if ((size_t)(2 * plane) * sizeof(float2) > kRtLds) return false;
+    if ((int64_t)ny * nx > (int64_t)kRtVox * kRtThreads) return false;
+    // z kernel: three column tiles and the two F tiles
+    if ((size_t)5 * nz * kRtTq * sizeof(float2) > kRtLds) return false;
+    return true;
+}
+
+int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t s) {
+    if (a.nrot <= 0) return kOk;
+    if (!plane_rt_supported(f.nz, f.ny, f.nx)) {
+        set_error("runtime-planned fused pipeline not available for grid (%d, %d, %d)", f.nz,
+                  f.ny, f.nx);
+        return kErrUnsupported;
+    }
+    if (a.nrot > 65535 || a.chunk_z < 1 || a.chunk_lcc < 1) {
+        set_error("fused: invalid batch or chunk sizes");
+        return kErrArg;
+    }
+    RtGeo g;
+    g.nz = f.nz;
+    g.ny = f.ny;
+    g.nx = f.nx;
+    g.M = f.nx / 2;
+    g.nxh = g.M + 1;
+    g.plane = f.ny * g.nxh;
+    g.H = (int64_t)f.nz * g.plane;
+    const size_t lds_plane = (size_t)2 * g.plane * sizeof(float2);
+    PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt, kRtLds));
+    PF_CHECK_STATUS(allow_lds((const void*)k_zcorr_rt, kRtLds));
+    PF_CHECK_STATUS(allow_lds((const void*)k_inv_plane_rt, kRtLds));
+    if (which == 0) {
+        const int ngrid = a.binary_mask ? 2 : 3;
+        hipLaunchKernelGGL(k_fwd_plane_rt, dim3(f.nz, a.nrot), dim3(kRtThreads), lds_plane, s,
+                           a.tm, a.rots, g, a.rmax, a.zr, ngrid, a.work, f.px, f.py, f.twx, f.twy,
+                           f.twfull);
+        return launched("k_fwd_plane_rt");
+    }
+    if (which == 1) {
+        const int tiles = (g.plane + kRtTq - 1) / kRtTq;
+        const size_t lds = (size_t)5 * f.nz * kRtTq * sizeof(float2);
+        hipLaunchKernelGGL(k_zcorr_rt, dim3(tiles, (a.nrot + a.chunk_z - 1) / a.chunk_z),
+                           dim3(kRtZThreads), lds, s, a.work, a.F1, a.F2, g, a.nrot, a.chunk_z,
+                           a.binary_mask ? 1 : 0, f.pz, f.twz);
+        return launched("k_zcorr_rt");
+    }
+    hipLaunchKernelGGL(k_inv_plane_rt, dim3(f.nz, (a.nrot + a.chunk_lcc - 1) / a.chunk_lcc),
+                       dim3(kRtThreads), lds_plane, s, a.work, a.lcc_mask, a.keys, g, a.nrot,
+                       a.chunk_lcc, a.first, a.norm, f.px, f.py, f.twx, f.twy, f.twfull);
+    return launched("k_inv_plane_rt");
+}
+
+}  // namespace pf

@@ -54,6 +54,11 @@ WORKLOADS = {
     "batch128": Workload("batch128", 128, 1.2, 12000, 5.0, 5.0, seed=3000),
     "search128": Workload("search128", 128, 1.2, 12000, 5.0, 10.0, seed=3000),
     "prec192": Workload("prec192", 192, 1.0, 25000, 4.0, 10.0, seed=4000),
+    # grids without a compiled fused size (the run-time planned pipeline):
+    # 120^3 and a non-cubic, radix-7 shape of the kind `extend` produces
+    "grid120": Workload("grid120", 120, 1.3, 9000, 5.5, 10.0, seed=6000),
+    "grid84x96x112": Workload("grid84x96x112", 84, 1.4, 6000, 6.0, 10.0, seed=6100,
+                              shape=(84, 96, 112)),
 }
 
 

@@ -52,11 +52,19 @@ def test_ctx_bytes_model():
         V * 25 + 2 * H * 8 + 3 * 72 + 4 * (((n // 8) ** 3 + 31) // 32) + 3 * 3 * H * 8
     assert lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["fused"], 0, 1) == \
         V * 25 + 2 * H * 8 + 72 + 4 * (((n // 8) ** 3 + 31) // 32) + 16 * V
-    # grids without a fused pipeline take the unfused work layout in auto mode
+    # grids without a compiled size take the run-time planned fused pipeline in
+    # auto mode (spectra only) where their planes fit, else the unfused layout
     n2 = 120 + 6
     V2, H2 = n2 ** 3, n2 * n2 * (n2 // 2 + 1)
     assert lib.pf_ctx_bytes(n2, n2, n2, b, _lib.MODES["auto"], 0, 0) == \
-        V2 * 25 + 2 * H2 * 8 + 3 * b * (H2 * 8 + V2 * 4)
+        V2 * 25 + 2 * H2 * 8 + 3 * b * H2 * 8
+    assert lib.pf_ctx_bytes(n2, n2, n2, b, _lib.MODES["fused_rt"], 0, 0) == \
+        V2 * 25 + 2 * H2 * 8 + 3 * b * H2 * 8
+    for shape in ((125, 125, 125), (200, 200, 200)):   # odd x; planes beyond LDS
+        nz, ny, nx = shape
+        V3, H3 = nz * ny * nx, nz * ny * (nx // 2 + 1)
+        assert lib.pf_ctx_bytes(nz, ny, nx, b, _lib.MODES["auto"], 0, 0) == \
+            V3 * 25 + 2 * H3 * 8 + 3 * b * (H3 * 8 + V3 * 4)
     assert lib.pf_ctx_bytes(0, 1, 1, 1, 0, 0, 0) == -1
     assert lib.pf_ctx_bytes(8, 8, 8, 1, 0, 0, -1) == -1
 

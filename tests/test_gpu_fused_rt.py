@@ -1,0 +1,120 @@
+"""GPU parity of the run-time planned fused pipeline (PF_MODE_FUSED_RT): grids
+without a compiled fused size -- 120^3 (8 * 3 * 5) and the non-cubic,
+radix-7 shapes the powerfit command line produces (``extend`` to the next
+2,3,5,7-smooth length per axis, powerfit.py:231-233) -- against the FP64
+oracle, with 0/1, core-weighted and Laplace-filtered inputs.
+
+Bar: best LCC within 1e-4 of the oracle; rotation indices identical where
+the oracle's winner leads by more than 1e-4.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import correlator as oc
+from powerfit_amd import HipCorrelator
+from powerfit_amd.rotations import rotations_for_angle
+from powerfit_amd.synth import Workload, make_case
+
+from parity import assert_scan_parity
+
+pytestmark = pytest.mark.gpu
+NPROC = min(8, os.cpu_count() or 1)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+CASES = {
+    "c120": Workload("c120", 120, 1.5, 4000, 6.0, 20.0, seed=8100),
+    "c120_lc": Workload("c120_lc", 120, 1.5, 4000, 6.0, 20.0, laplace=True, core_weighted=True,
+                        seed=8101),
+    "s84x96x112": Workload("s84x96x112", 84, 1.6, 3000, 6.5, 20.0, seed=8102,
+                           shape=(84, 96, 112)),
+    "s70x72x98_cw": Workload("s70x72x98_cw", 70, 1.8, 2000, 7.0, 20.0, core_weighted=True,
+                             seed=8103, shape=(70, 72, 98)),
+    "s50x60x42_l": Workload("s50x60x42_l", 42, 2.0, 800, 8.0, 20.0, laplace=True, seed=8104,
+                            shape=(50, 60, 42)),
+}
+
+
+def scan(c, R, mode, batch=8, **kw):
+    corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode=mode, batch=batch,
+                         **kw)
+    corr.template = c.template
+    corr.mask = c.mask
+    corr.rotations = R
+    corr.scan()
+    return corr
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fused_rt_vs_oracle(name):
+    w = CASES[name]
+    c = make_case(w)
+    R = rotations_for_angle(w.angle)[:16]
+    say(f"scan {name} {c.target.shape} fused_rt, 16 rotations")
+    corr = scan(c, R, "fused_rt")
+    assert corr.fused_rt_available and not corr.fused_available
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
+                           laplace=c.laplace, nproc=NPROC)
+    err, dec = assert_scan_parity(corr.lcc, corr.rot, ref)
+    say(f"  max err {err:.2e}, decisive voxels {dec}")
+
+
+def test_auto_takes_fused_rt_and_batches_agree():
+    """AUTO runs the run-time planned kernels where no compiled size exists;
+    batch and chunk sizes and rotation ranges do not change a result."""
+    c = make_case(CASES["s84x96x112"])
+    R = rotations_for_angle(20.0)[:12]
+    say("s84x96x112: auto vs fused_rt at batches 12 / 5, ranges [0, 7) + [7, 12)")
+    a = scan(c, R, "auto", batch=12)
+    b = scan(c, R, "fused_rt", batch=5)
+    assert np.array_equal(a.lcc, b.lcc) and np.array_equal(a.rot, b.rot)
+    b.set_tuning(1, 1)
+    b.reset()
+    b.enqueue_scan(0, 7)
+    b.enqueue_scan(7, 5)
+    lcc, rot = b.result()
+    assert np.array_equal(lcc, a.lcc) and np.array_equal(rot, a.rot)
+
+
+def test_fused_rt_without_binary_shortcut():
+    """0/1 mask with W^2 transformed as its own grid (the weighted form)."""
+    c = make_case(CASES["c120"])
+    R = rotations_for_angle(20.0)[:8]
+    say("c120 fused_rt, binary shortcut off, 8 rotations")
+    corr = scan(c, R, "fused_rt", binary_shortcut=False)
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
+                           laplace=c.laplace, nproc=NPROC)
+    assert_scan_parity(corr.lcc, corr.rot, ref)
+
+
+def test_fused_rt_on_a_compiled_size_matches_oracle():
+    """Forced onto a grid that also has compiled kernels (64^3): the same
+    results within the tolerance."""
+    c = make_case(Workload("rt64", 64, 2.0, 1500, 8.0, 10.0, seed=8105))
+    R = rotations_for_angle(10.0)[:16]
+    say("64^3 fused_rt vs compiled fused vs oracle, 16 rotations")
+    rt = scan(c, R, "fused_rt")
+    fu = scan(c, R, "fused")
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R, nproc=NPROC)
+    assert_scan_parity(rt.lcc, rt.rot, ref)
+    assert np.abs(rt.lcc - fu.lcc).max() <= 1e-4
+
+
+def test_odd_x_length_falls_back_to_batched():
+    """Odd nx has no half-length x transform: AUTO runs the unfused kernels,
+    fused_rt is refused."""
+    c = make_case(Workload("odd", 45, 2.0, 600, 8.0, 20.0, seed=8106, shape=(48, 50, 45)))
+    R = rotations_for_angle(20.0)[:6]
+    corr = scan(c, R, "auto")
+    assert not corr.fused_rt_available and not corr.fused_available
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R, nproc=NPROC)
+    assert_scan_parity(corr.lcc, corr.rot, ref)
+    with pytest.raises(RuntimeError):
+        scan(c, R, "fused_rt")
