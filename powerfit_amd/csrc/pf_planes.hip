@@ -22,6 +22,8 @@
 //
 // Limits (plane_rt_supported): even nx; two planes of ny x (nx/2+1) complex
 // in LDS; at most kRtVox voxels of a plane per thread of the LCC stage.
+#include <algorithm>
+
 #include "pf_internal.h"
 #include "pf_rotate.h"
 
@@ -83,20 +85,137 @@ __device__ __forceinline__ void gather_pw_rt(const float2* __restrict__ tm, int 
     *w = uz ? n1.y : n0.y;
 }
 
+// One Stockham butterfly of radix R (the stage of pf_fft.h) with its twiddle
+// phase k = j mod Ns and twiddle step n / (Ns R) passed in, so a butterfly
+// needs no integer division.
+template <int R, int DIR>
+__device__ __forceinline__ void bfly_k(const float2* sp, float2* dp, int j, int k, int Ns, int nb,
+                                       int step, int es, const float2* tw) {
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = sp[(j + r * nb) * es];
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<DIR>(tw, r * k * step));
+    dft_fixed<R, DIR>(v);
+    const int od = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dp[(od + r * Ns) * es] = v[r];
+}
+
+template <int DIR>
+__device__ __noinline__ void bfly_k_any(const float2* sp, float2* dp, int R, int j, int k, int Ns,
+                                        int nb, int step, int es, const float2* tw) {
+    const int od = (j - k) * R + k;
+    for (int q = 0; q < R; ++q) {
+        float2 acc = c2(0.f, 0.f);
+        for (int m = 0; m < R; ++m) {
+            float2 x = sp[(j + m * nb) * es];
+            if (m > 0) x = cmul(x, twid<DIR>(tw, m * k * step));
+            const int e = (m * q) % R;
+            acc = cadd(acc, e ? cmul(x, twid<DIR>(tw, e * nb)) : x);
+        }
+        dp[(od + q * Ns) * es] = acc;
+    }
+}
+
+// lds_fft (pf_fft.h) with each thread stepping through its butterflies by the
+// block size: (transform f, butterfly j) and k = j mod Ns advance by adds and
+// compares (nb is a multiple of Ns, so wrapping j past nb keeps k), and the
+// twiddles are read from LDS.  Returns the buffer holding the result; ends
+// with a barrier.
+template <int DIR>
+__device__ float2* lds_fft_rt(float2* a, float2* b, const Plan1D& p, const float2* tw, int nfft,
+                              int fs, int es) {
+    const int n = p.n, T = blockDim.x, t = threadIdx.x;
+    int Ns = 1;
+    for (int s = 0; s < p.nstages; ++s) {
+        const int R = p.radix[s];
+        const int nb = n / R, step = n / (Ns * R), total = nfft * nb;
+        int f, j, df, dj;
+        if (es == 1) {   // u = f nb + j
+            f = t / nb;
+            j = t - f * nb;
+            df = T / nb;
+            dj = T - df * nb;
+        } else {         // u = j nfft + f
+            j = t / nfft;
+            f = t - j * nfft;
+            dj = T / nfft;
+            df = T - dj * nfft;
+        }
+        int k = j % Ns;
+        const int dk = dj % Ns;
+        for (int u = t; u < total; u += T) {
+            const float2* sp = a + f * fs;
+            float2* dp = b + f * fs;
+            switch (R) {
+                case 2: bfly_k<2, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
+                case 3: bfly_k<3, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
+                case 4: bfly_k<4, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
+                case 5: bfly_k<5, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
+                case 8: bfly_k<8, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
+                default: bfly_k_any<DIR>(sp, dp, R, j, k, Ns, nb, step, es, tw); break;
+            }
+            k += dk;
+            if (k >= Ns) k -= Ns;
+            if (es == 1) {
+                j += dj;
+                f += df;
+                if (j >= nb) {
+                    j -= nb;
+                    ++f;
+                }
+            } else {
+                f += df;
+                j += dj;
+                if (f >= nfft) {
+                    f -= nfft;
+                    ++j;
+                    if (++k >= Ns) k -= Ns;
+                }
+            }
+        }
+        __syncthreads();
+        float2* tmp = a;
+        a = b;
+        b = tmp;
+        Ns *= R;
+    }
+    return a;
+}
+
+// Copy a twiddle table into LDS.
+__device__ __forceinline__ void stage_tw(float2* dst, const float2* __restrict__ src, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
 struct RtGeo {
     int nz, ny, nx, M, nxh, plane;   // M = nx / 2, nxh = M + 1, plane = ny * nxh
     int64_t H;
 };
 
-// Forward stage.  A and B are two LDS planes of ny x nxh complex.
+// LDS of a plane kernel: two planes of ny x nxh complex, then the twiddles
+// of the x (M), y (ny) and R2C/C2R (nx) tables.
+__host__ __device__ inline size_t plane_lds_bytes(const RtGeo& g) {
+    return sizeof(float2) * ((size_t)2 * g.plane + g.M + g.ny + g.nx);
+}
+
+// Forward stage.  Planes zlo, zlo + 1, ... (mod nz) of the support.
 __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
     const float2* __restrict__ tm, const double* __restrict__ rots, RtGeo g, int r, int zr,
-    int ngrid, float2* __restrict__ out, Plan1D px, Plan1D py, const float2* __restrict__ twx,
-    const float2* __restrict__ twy, const float2* __restrict__ twfull) {
+    int zlo, int ngrid, float2* __restrict__ out, Plan1D px, Plan1D py,
+    const float2* __restrict__ twx_g, const float2* __restrict__ twy_g,
+    const float2* __restrict__ twf_g) {
     extern __shared__ float2 lds[];
     float2* const A = lds;
     float2* const B = lds + g.plane;
-    const int iz = blockIdx.x, b = blockIdx.y;
+    float2* const twx = lds + 2 * g.plane;
+    float2* const twy = twx + g.M;
+    float2* const twfull = twy + g.ny;
+    stage_tw(twx, twx_g, g.M);
+    stage_tw(twy, twy_g, g.ny);
+    stage_tw(twfull, twf_g, g.nx);
+    const int iz = (blockIdx.x + zlo + g.nz) % g.nz, b = blockIdx.y;
     const int sz = scoord(iz, r, g.nz);
     // gathered voxels: inside the rotation sphere and the support ball
     const int rg = r < zr ? r : zr, rg2 = rg * rg;
@@ -125,7 +244,7 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
             A[u] = c2(v2[0], v2[1]);
         }
         __syncthreads();
-        float2* Z = lds_fft<-1>(A, B, px, twx, g.ny, g.M, 1);
+        float2* Z = lds_fft_rt<-1>(A, B, px, twx, g.ny, g.M, 1);
         float2* Xp = Z == A ? B : A;
         // R2C post-processing, rows of nxh: X[k] = E[k] + w^k O[k]
         for (int u = threadIdx.x; u < g.plane; u += blockDim.x) {
@@ -145,7 +264,7 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
         }
         __syncthreads();
         // y FFT of the nxh columns (element stride nxh)
-        const float2* Y = lds_fft<-1>(Xp, Z, py, twy, g.nxh, 1, g.nxh);
+        const float2* Y = lds_fft_rt<-1>(Xp, Z, py, twy, g.nxh, 1, g.nxh);
         float2* o = dst + (int64_t)gi * g.H;
         for (int u = threadIdx.x; u < g.plane; u += blockDim.x) o[u] = Y[u];
         __syncthreads();
@@ -156,7 +275,7 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
 // and the F1, F2 tiles.  BIN: W's spectrum gives both A (F1) and B (F2).
 __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
     float2* __restrict__ X, const float2* __restrict__ F1, const float2* __restrict__ F2,
-    RtGeo g, int nrot, int chunk, int bin, Plan1D pz, const float2* __restrict__ twz) {
+    RtGeo g, int nrot, int chunk, int bin, int zr, Plan1D pz, const float2* __restrict__ twz_g) {
     extern __shared__ float2 lds[];
     const int tile = g.nz * kRtTq;
     float2* const S0 = lds;
@@ -164,6 +283,8 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
     float2* const S2 = lds + 2 * tile;
     float2* const T1 = lds + 3 * tile;
     float2* const T2 = lds + 4 * tile;
+    float2* const twz = lds + 5 * tile;
+    stage_tw(twz, twz_g, g.nz);
     const int q0 = blockIdx.x * kRtTq;
     const int nc = min(kRtTq, g.plane - q0);
     const int b0 = blockIdx.y * chunk, b1 = min(nrot, b0 + chunk);
@@ -173,10 +294,14 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
         T1[z * kRtTq + c] = F1[(int64_t)z * g.plane + q0 + c];
         T2[z * kRtTq + c] = F2[(int64_t)z * g.plane + q0 + c];
     }
+    // a forward grid's column tile; planes beyond the support were not
+    // written by the forward stage: they are zero
     auto load = [&](const float2* src, float2* d) {
         for (int u = threadIdx.x; u < n; u += blockDim.x) {
             const int z = u / nc, c = u - z * nc;
-            d[z * kRtTq + c] = src[(int64_t)z * g.plane + q0 + c];
+            const int sz = z <= g.nz / 2 ? z : z - g.nz;
+            d[z * kRtTq + c] = (sz >= -zr && sz <= zr) ? src[(int64_t)z * g.plane + q0 + c]
+                                                       : c2(0.f, 0.f);
         }
         __syncthreads();
     };
@@ -201,30 +326,30 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
         float2* base = X + (int64_t)b * 3 * g.H;
         // G = IFFT(conj(FP) F1)
         load(base, S0);
-        float2* Z = lds_fft<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
+        float2* Z = lds_fft_rt<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
         mul(Z, T1, nullptr, nullptr);
-        const float2* Zi = lds_fft<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
+        const float2* Zi = lds_fft_rt<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
         store(Zi, base);
         __syncthreads();
         // A = IFFT(conj(FW) F1); B = IFFT(conj(FW) F2) or IFFT(conj(FW^2) F2)
         load(base + g.H, S0);
-        Z = lds_fft<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
+        Z = lds_fft_rt<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
         if (bin) {
             mul(Z, T1, S2, T2);
-            Zi = lds_fft<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
+            Zi = lds_fft_rt<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
             store(Zi, base + g.H);
             __syncthreads();
-            Zi = lds_fft<1>(S2, S0, pz, twz, nc, 1, kRtTq);
+            Zi = lds_fft_rt<1>(S2, S0, pz, twz, nc, 1, kRtTq);
             store(Zi, base + 2 * g.H);
         } else {
             mul(Z, T1, nullptr, nullptr);
-            Zi = lds_fft<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
+            Zi = lds_fft_rt<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
             store(Zi, base + g.H);
             __syncthreads();
             load(base + 2 * g.H, S0);
-            Z = lds_fft<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
+            Z = lds_fft_rt<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
             mul(Z, T2, nullptr, nullptr);
-            Zi = lds_fft<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
+            Zi = lds_fft_rt<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
             store(Zi, base + 2 * g.H);
         }
         __syncthreads();
@@ -237,11 +362,17 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
 __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
     const float2* __restrict__ X, const uint8_t* __restrict__ lmask,
     unsigned long long* __restrict__ keys, RtGeo g, int nrot, int chunk, int64_t first,
-    double norm, Plan1D px, Plan1D py, const float2* __restrict__ twx,
-    const float2* __restrict__ twy, const float2* __restrict__ twfull) {
+    double norm, Plan1D px, Plan1D py, const float2* __restrict__ twx_g,
+    const float2* __restrict__ twy_g, const float2* __restrict__ twf_g) {
     extern __shared__ float2 lds[];
     float2* const A = lds;
     float2* const B = lds + g.plane;
+    float2* const twx = lds + 2 * g.plane;
+    float2* const twy = twx + g.M;
+    float2* const twfull = twy + g.ny;
+    stage_tw(twx, twx_g, g.M);
+    stage_tw(twy, twy_g, g.ny);
+    stage_tw(twfull, twf_g, g.nx);
     const int iz = blockIdx.x;
     const int b0 = blockIdx.y * chunk, b1 = min(nrot, b0 + chunk);
     const int nvox = g.ny * g.nx;
@@ -260,7 +391,7 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
             for (int u = threadIdx.x; u < g.plane; u += blockDim.x) A[u] = src[u];
             __syncthreads();
             // inverse y over the nxh columns
-            float2* Y = lds_fft<1>(A, B, py, twy, g.nxh, 1, g.nxh);
+            float2* Y = lds_fft_rt<1>(A, B, py, twy, g.nxh, 1, g.nxh);
             float2* D = Y == A ? B : A;
             // C2R pre-processing, rows of M: z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
             const int rowM = g.ny * g.M;
@@ -277,7 +408,7 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
                 D[u] = cadd(cadd(xk, xmc), mul_si<1>(d));
             }
             __syncthreads();
-            const float2* Z = lds_fft<1>(D, Y, px, twx, g.ny, g.M, 1);
+            const float2* Z = lds_fft_rt<1>(D, Y, px, twx, g.ny, g.M, 1);
             const float* zr = reinterpret_cast<const float*>(Z);   // row y: 2 M reals
 #pragma unroll
             for (int k = 0; k < kRtVox; ++k) {
@@ -327,11 +458,15 @@ int allow_lds(const void* fn, size_t bytes) {
 
 bool plane_rt_supported(int nz, int ny, int nx) {
     if (nx % 2 != 0 || nz < 2 || ny < 2 || nx < 4) return false;
-    const int64_t plane = (int64_t)ny * (nx / 2 + 1);
-    if ((size_t)(2 * plane) * sizeof(float2) > kRtLds) return false;
+    RtGeo g;
+    g.ny = ny;
+    g.nx = nx;
+    g.M = nx / 2;
+    g.plane = ny * (g.M + 1);
+    if (plane_lds_bytes(g) > kRtLds) return false;
     if ((int64_t)ny * nx > (int64_t)kRtVox * kRtThreads) return false;
-    // z kernel: three column tiles and the two F tiles
-    if ((size_t)5 * nz * kRtTq * sizeof(float2) > kRtLds) return false;
+    // z kernel: three column tiles, the two F tiles and the z twiddles
+    if ((size_t)(5 * kRtTq + 1) * nz * sizeof(float2) > kRtLds) return false;
     return true;
 }
 
@@ -354,23 +489,27 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
     g.nxh = g.M + 1;
     g.plane = f.ny * g.nxh;
     g.H = (int64_t)f.nz * g.plane;
-    const size_t lds_plane = (size_t)2 * g.plane * sizeof(float2);
+    const size_t lds_plane = plane_lds_bytes(g);
     PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt, kRtLds));
     PF_CHECK_STATUS(allow_lds((const void*)k_zcorr_rt, kRtLds));
     PF_CHECK_STATUS(allow_lds((const void*)k_inv_plane_rt, kRtLds));
+    // planes |z| <= zr only, the z stage reading the others as zero
+    const int zr = std::max(0, std::min(a.zr, f.nz / 2));
+    const bool skip = 2 * zr + 1 < f.nz;
     if (which == 0) {
         const int ngrid = a.binary_mask ? 2 : 3;
-        hipLaunchKernelGGL(k_fwd_plane_rt, dim3(f.nz, a.nrot), dim3(kRtThreads), lds_plane, s,
-                           a.tm, a.rots, g, a.rmax, a.zr, ngrid, a.work, f.px, f.py, f.twx, f.twy,
-                           f.twfull);
+        const int nplanes = skip ? 2 * zr + 1 : f.nz;
+        hipLaunchKernelGGL(k_fwd_plane_rt, dim3(nplanes, a.nrot), dim3(kRtThreads), lds_plane, s,
+                           a.tm, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, ngrid, a.work, f.px,
+                           f.py, f.twx, f.twy, f.twfull);
         return launched("k_fwd_plane_rt");
     }
     if (which == 1) {
         const int tiles = (g.plane + kRtTq - 1) / kRtTq;
-        const size_t lds = (size_t)5 * f.nz * kRtTq * sizeof(float2);
+        const size_t lds = (size_t)(5 * kRtTq + 1) * f.nz * sizeof(float2);
         hipLaunchKernelGGL(k_zcorr_rt, dim3(tiles, (a.nrot + a.chunk_z - 1) / a.chunk_z),
                            dim3(kRtZThreads), lds, s, a.work, a.F1, a.F2, g, a.nrot, a.chunk_z,
-                           a.binary_mask ? 1 : 0, f.pz, f.twz);
+                           a.binary_mask ? 1 : 0, skip ? zr : f.nz, f.pz, f.twz);
         return launched("k_zcorr_rt");
     }
     hipLaunchKernelGGL(k_inv_plane_rt, dim3(f.nz, (a.nrot + a.chunk_lcc - 1) / a.chunk_lcc),
