@@ -486,6 +486,7 @@ int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** o
         if (c->num_cu <= 0) c->num_cu = 256;
     }
     if (!s) s = fft3d_init(&c->fft, nz, ny, nx);
+    if (!s && plane_rt_supported(nz, ny, nx)) s = prepare_planes_rt();
     if (!s && (flags & PF_FLAG_SPLIT_DFT)) s = fft3d_set_split(&c->fft, true);
     if (!s && hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");

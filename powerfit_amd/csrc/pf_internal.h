@@ -132,6 +132,9 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s);
 int launch_pack_tm(const float* tmpl, const float* mask, float2* tm, int64_t n, hipStream_t s);
 // ---- runtime-planned fused path (pf_planes.hip): any grid within its limits ----
 bool plane_rt_supported(int nz, int ny, int nx);
+// Let the run-time planned kernels take their LDS (function attributes; done
+// once per context creation, outside any stream capture).
+int prepare_planes_rt();
 // Stage `which` (0, 1, 2 as launch_fused) of the runtime-planned pipeline; uses
 // f's plans and twiddles and a's buffers and options (a.n, a.occ and
 // a.skip_planes are not used: every plane is transformed).

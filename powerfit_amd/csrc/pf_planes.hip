@@ -31,8 +31,8 @@ namespace pf {
 
 namespace {
 
-constexpr int kRtThreads = 512;          // plane kernels
-constexpr int kRtVox = 32;               // voxels of a plane owned per thread (K3)
+constexpr int kRtThreads = 1024;         // plane kernels (one block per CU: the LDS)
+constexpr int kRtVox = 16;               // voxels of a plane owned per thread (K3)
 constexpr int kRtZThreads = 256;         // z kernel
 constexpr int kRtTq = 16;                // z columns per block
 constexpr size_t kRtLds = 150 * 1024;    // dynamic LDS a plane kernel may take
@@ -272,7 +272,8 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
 }
 
 // z stage.  LDS: S0, S1, S2 (nz x TQ complex each, element (z, c) at z TQ + c)
-// and the F1, F2 tiles.  BIN: W's spectrum gives both A (F1) and B (F2).
+// and the z twiddles; the F1, F2 tiles are read from global memory (L2) by
+// each multiply.  BIN: W's spectrum gives both A (F1) and B (F2).
 __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
     float2* __restrict__ X, const float2* __restrict__ F1, const float2* __restrict__ F2,
     RtGeo g, int nrot, int chunk, int bin, int zr, Plan1D pz, const float2* __restrict__ twz_g) {
@@ -281,19 +282,14 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
     float2* const S0 = lds;
     float2* const S1 = lds + tile;
     float2* const S2 = lds + 2 * tile;
-    float2* const T1 = lds + 3 * tile;
-    float2* const T2 = lds + 4 * tile;
-    float2* const twz = lds + 5 * tile;
+    float2* const twz = lds + 3 * tile;
     stage_tw(twz, twz_g, g.nz);
     const int q0 = blockIdx.x * kRtTq;
     const int nc = min(kRtTq, g.plane - q0);
     const int b0 = blockIdx.y * chunk, b1 = min(nrot, b0 + chunk);
     const int n = g.nz * nc;
-    for (int u = threadIdx.x; u < n; u += blockDim.x) {
-        const int z = u / nc, c = u - z * nc;
-        T1[z * kRtTq + c] = F1[(int64_t)z * g.plane + q0 + c];
-        T2[z * kRtTq + c] = F2[(int64_t)z * g.plane + q0 + c];
-    }
+    const float2* T1 = F1 + q0;
+    const float2* T2 = F2 + q0;
     // a forward grid's column tile; planes beyond the support were not
     // written by the forward stage: they are zero
     auto load = [&](const float2* src, float2* d) {
@@ -311,14 +307,16 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
             d[(int64_t)z * g.plane + q0 + c] = s[z * kRtTq + c];
         }
     };
-    // in place: s = conj(s) * t (and, with t2, d = conj(s) * t2 first)
+    // in place: s = conj(s) * t (and, with t2, d = conj(s) * t2 first); t, t2
+    // are the target spectra at this tile's columns (row stride plane)
     auto mul = [&](float2* s, const float2* t, float2* d, const float2* t2) {
         for (int u = threadIdx.x; u < n; u += blockDim.x) {
             const int z = u / nc, c = u - z * nc;
             const int i = z * kRtTq + c;
+            const int64_t gi = (int64_t)z * g.plane + c;
             const float2 x = cconj(s[i]);
-            if (d) d[i] = cmul(x, t2[i]);
-            s[i] = cmul(x, t[i]);
+            if (d) d[i] = cmul(x, t2[gi]);
+            s[i] = cmul(x, t[gi]);
         }
         __syncthreads();
     };
@@ -378,6 +376,7 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
     const int nvox = g.ny * g.nx;
     const double nv = norm * ((double)g.nz * g.ny * g.nx);   // N V (exact)
     const int64_t vz = (int64_t)iz * nvox;
+    static_assert(kRtVox <= 32, "mask bits of the owned voxels fit a 32-bit word");
     uint32_t mbits = 0;
 #pragma unroll
     for (int k = 0; k < kRtVox; ++k) {
@@ -465,9 +464,16 @@ bool plane_rt_supported(int nz, int ny, int nx) {
     g.plane = ny * (g.M + 1);
     if (plane_lds_bytes(g) > kRtLds) return false;
     if ((int64_t)ny * nx > (int64_t)kRtVox * kRtThreads) return false;
-    // z kernel: three column tiles, the two F tiles and the z twiddles
-    if ((size_t)(5 * kRtTq + 1) * nz * sizeof(float2) > kRtLds) return false;
+    // z kernel: three column tiles and the z twiddles
+    if ((size_t)(3 * kRtTq + 1) * nz * sizeof(float2) > kRtLds) return false;
     return true;
+}
+
+int prepare_planes_rt() {
+    PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt, kRtLds));
+    PF_CHECK_STATUS(allow_lds((const void*)k_zcorr_rt, kRtLds));
+    PF_CHECK_STATUS(allow_lds((const void*)k_inv_plane_rt, kRtLds));
+    return kOk;
 }
 
 int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t s) {
@@ -490,9 +496,6 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
     g.plane = f.ny * g.nxh;
     g.H = (int64_t)f.nz * g.plane;
     const size_t lds_plane = plane_lds_bytes(g);
-    PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt, kRtLds));
-    PF_CHECK_STATUS(allow_lds((const void*)k_zcorr_rt, kRtLds));
-    PF_CHECK_STATUS(allow_lds((const void*)k_inv_plane_rt, kRtLds));
     // planes |z| <= zr only, the z stage reading the others as zero
     const int zr = std::max(0, std::min(a.zr, f.nz / 2));
     const bool skip = 2 * zr + 1 < f.nz;
@@ -506,7 +509,7 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
     }
     if (which == 1) {
         const int tiles = (g.plane + kRtTq - 1) / kRtTq;
-        const size_t lds = (size_t)(5 * kRtTq + 1) * f.nz * sizeof(float2);
+        const size_t lds = (size_t)(3 * kRtTq + 1) * f.nz * sizeof(float2);
         hipLaunchKernelGGL(k_zcorr_rt, dim3(tiles, (a.nrot + a.chunk_z - 1) / a.chunk_z),
                            dim3(kRtZThreads), lds, s, a.work, a.F1, a.F2, g, a.nrot, a.chunk_z,
                            a.binary_mask ? 1 : 0, skip ? zr : f.nz, f.pz, f.twz);

@@ -59,6 +59,8 @@ WORKLOADS = {
     "grid120": Workload("grid120", 120, 1.3, 9000, 5.5, 10.0, seed=6000),
     "grid84x96x112": Workload("grid84x96x112", 84, 1.4, 6000, 6.0, 10.0, seed=6100,
                               shape=(84, 96, 112)),
+    "grid70x72x98": Workload("grid70x72x98", 70, 1.6, 4000, 6.5, 10.0, seed=6200,
+                             shape=(70, 72, 98)),
 }
 
 

@@ -118,3 +118,25 @@ def test_odd_x_length_falls_back_to_batched():
     assert_scan_parity(corr.lcc, corr.rot, ref)
     with pytest.raises(RuntimeError):
         scan(c, R, "fused_rt")
+
+
+def test_fused_rt_graph_and_noalloc_equal_eager():
+    """The run-time planned stages under a captured graph and through the
+    allocation-free entry point give the eager scan's maps bit for bit."""
+    from powerfit_amd import _lib
+    c = make_case(CASES["s84x96x112"])
+    R = rotations_for_angle(20.0)[:10]
+    say("s84x96x112 fused_rt: graph replays and pf_scan_noalloc vs eager")
+    corr = scan(c, R, "fused_rt", batch=4)
+    eager = (corr.lcc.copy(), corr.rot.copy())
+    for _ in range(2):
+        corr.reset()
+        corr.enqueue_scan_graph(0, 10)
+        lcc, rot = corr.result()
+        assert np.array_equal(lcc, eager[0]) and np.array_equal(rot, eager[1])
+    lib = _lib.load()
+    corr.reset()
+    _lib.check(lib.pf_scan_noalloc(corr._ctx, 0, 10, 4, _lib.MODES["fused_rt"]),
+               "pf_scan_noalloc")
+    lcc, rot = corr.result()
+    assert np.array_equal(lcc, eager[0]) and np.array_equal(rot, eager[1])
