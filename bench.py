@@ -187,6 +187,24 @@ def precision_report(args, HipCorrelator, case, R, w):
     return out
 
 
+def device_copy_gbs(torch, mib=1024, reps=10):
+    """Measured HBM bandwidth of a plain device-to-device copy (bytes read +
+    written per second), the practical ceiling the roofline peak is set
+    against."""
+    n = (mib << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t
+    del a, b
+    return 2.0 * (mib << 20) * reps / secs / 1e9
+
+
 def serving_bench(args):
     """Serving configuration: a Poisson stream of mixed-size fits through
     FitServer (throughput, p50/p99 latency overall and per grid size), then
@@ -366,6 +384,7 @@ def main(argv=None):
     dom_bytes = per_rot * rot_done + per_launch * dom_n
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     path_gbs = value * path_bytes_per_rotation(V, H) / 1e9
+    copy_gbs = device_copy_gbs(torch)
 
     # plain one-rotation-per-launch path on the same inputs
     plain_value = None
@@ -449,7 +468,8 @@ def main(argv=None):
                      "alg_bytes_per_rotation": per_rot, "alg_bytes_per_launch_fixed": per_launch,
                      "alg_bytes_per_launch": dom_bytes / dom_n,
                      "traffic": measured_traffic(args.workload, args.mode, dom, rot_done / dom_n,
-                                                 binary)},
+                                                 binary),
+                     "device_copy_gbs": copy_gbs, "frac_of_device_copy": achieved / copy_gbs},
         "path_roofline": {"bytes_per_rotation": path_bytes_per_rotation(V, H),
                           "achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": path_gbs / HBM_PEAK_GBS},
