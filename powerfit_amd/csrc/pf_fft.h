@@ -1,7 +1,7 @@
 // Mixed-radix FFT building blocks shared by every FFT kernel.
 //
 // A length-n transform is run as a sequence of Stockham autosort stages
-// (radix 2, 3, 4, 5, 8 in closed form, any other prime by an O(p^2) DFT),
+// (radix 2, 3, 4, 5, 7, 8 in closed form, any other prime by an O(p^2) DFT),
 // each stage reading one buffer and writing the other so that every
 // butterfly is independent within a stage.  The per-butterfly function is
 // __host__ __device__: the kernels call it from LDS, and a host build runs
@@ -29,7 +29,7 @@ struct Plan1D {
 };
 
 // Factor n into stages, largest closed-form radices first: 8, 4, 2, 3, 5,
-// then remaining primes.  Returns false when n < 1.
+// then remaining primes (7 in closed form, larger ones generic).  Returns false when n < 1.
 inline bool make_plan(int n, Plan1D* p) {
     if (n < 1) return false;
     p->n = n;
@@ -131,11 +131,44 @@ template <int DIR> PF_HD void dft8(float2* v) {
     }
 }
 
+// 7 points from the pairs t_n = v_n + v_(7-n), s_n = v_n - v_(7-n) (n = 1..3):
+// X_k = a_k + DIR i b_k and X_(7-k) = a_k - DIR i b_k with
+// a_k = v_0 + sum_n cos(2 pi k n / 7) t_n, b_k = sum_n sin(2 pi k n / 7) s_n.
+template <int DIR> PF_HD void dft7(float2* v) {
+    const float C1 = 0.62348980185873353053f, C2 = -0.22252093395631440429f;
+    const float C3 = -0.90096886790241912624f;
+    const float S1 = 0.78183148246802980871f, S2 = 0.97492791218182360702f;
+    const float S3 = 0.43388373911755812048f;
+    const float2 t1 = cadd(v[1], v[6]), t2 = cadd(v[2], v[5]), t3 = cadd(v[3], v[4]);
+    const float2 s1 = csub(v[1], v[6]), s2 = csub(v[2], v[5]), s3 = csub(v[3], v[4]);
+    const float2 x0 = v[0];
+    const float2 a1 = c2(x0.x + C1 * t1.x + C2 * t2.x + C3 * t3.x,
+                         x0.y + C1 * t1.y + C2 * t2.y + C3 * t3.y);
+    const float2 a2 = c2(x0.x + C2 * t1.x + C3 * t2.x + C1 * t3.x,
+                         x0.y + C2 * t1.y + C3 * t2.y + C1 * t3.y);
+    const float2 a3 = c2(x0.x + C3 * t1.x + C1 * t2.x + C2 * t3.x,
+                         x0.y + C3 * t1.y + C1 * t2.y + C2 * t3.y);
+    const float2 b1 = mul_si<DIR>(c2(S1 * s1.x + S2 * s2.x + S3 * s3.x,
+                                     S1 * s1.y + S2 * s2.y + S3 * s3.y));
+    const float2 b2 = mul_si<DIR>(c2(S2 * s1.x - S3 * s2.x - S1 * s3.x,
+                                     S2 * s1.y - S3 * s2.y - S1 * s3.y));
+    const float2 b3 = mul_si<DIR>(c2(S3 * s1.x - S1 * s2.x + S2 * s3.x,
+                                     S3 * s1.y - S1 * s2.y + S2 * s3.y));
+    v[0] = cadd(x0, cadd(t1, cadd(t2, t3)));
+    v[1] = cadd(a1, b1);
+    v[6] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[5] = csub(a2, b2);
+    v[3] = cadd(a3, b3);
+    v[4] = csub(a3, b3);
+}
+
 template <int R, int DIR> PF_HD void dft_fixed(float2* v) {
     if (R == 2) dft2<DIR>(v);
     else if (R == 3) dft3<DIR>(v);
     else if (R == 4) dft4<DIR>(v);
     else if (R == 5) dft5<DIR>(v);
+    else if (R == 7) dft7<DIR>(v);
     else if (R == 8) dft8<DIR>(v);
 }
 
@@ -158,7 +191,7 @@ PF_HD void stage_bfly(const float2* sp, float2* dp, int j, int Ns, int n, int es
     for (int r = 0; r < R; ++r) dp[(od + r * Ns) * es] = v[r];
 }
 
-// Stockham butterfly for any radix R (O(R^2)), used for primes >= 7.
+// Stockham butterfly for any radix R (O(R^2)), used for primes >= 11.
 template <int DIR>
 PF_HD void stage_bfly_generic(const float2* sp, float2* dp, int R, int j, int Ns, int n,
                               int es, const float2* tw) {
@@ -186,6 +219,7 @@ PF_HD void stage_dispatch(const float2* sp, float2* dp, int R, int j, int Ns, in
         case 3: stage_bfly<3, DIR>(sp, dp, j, Ns, n, es, tw); break;
         case 4: stage_bfly<4, DIR>(sp, dp, j, Ns, n, es, tw); break;
         case 5: stage_bfly<5, DIR>(sp, dp, j, Ns, n, es, tw); break;
+        case 7: stage_bfly<7, DIR>(sp, dp, j, Ns, n, es, tw); break;
         case 8: stage_bfly<8, DIR>(sp, dp, j, Ns, n, es, tw); break;
         default: stage_bfly_generic<DIR>(sp, dp, R, j, Ns, n, es, tw); break;
     }

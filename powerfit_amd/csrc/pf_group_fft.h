@@ -188,23 +188,7 @@ PF_HD void dft_any(float2* v) {
     else if constexpr (R == 12) dft_ab<4, 3, DIR>(v);
     else if constexpr (R == 20) dft_ab<4, 5, DIR>(v);
     else if constexpr (R == 24) dft_ab<8, 3, DIR>(v);
-    else if constexpr (R == 7) {
-        // 7-point DFT directly (rare radix)
-        float2 x[7];
-#pragma unroll
-        for (int i = 0; i < 7; ++i) x[i] = v[i];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            float2 acc = x[0];
-#pragma unroll
-            for (int n = 1; n < 7; ++n) {
-                const int m = (n * k) % 7;
-                const double a = (double)DIR * 2.0 * 3.14159265358979323846 * m / 7.0;
-                acc = cadd(acc, cmul(x[n], c2((float)__builtin_cos(a), (float)__builtin_sin(a))));
-            }
-            v[k] = acc;
-        }
-    } else dft_fixed<R, DIR>(v);
+    else dft_fixed<R, DIR>(v);
 }
 
 #if defined(__HIP__)

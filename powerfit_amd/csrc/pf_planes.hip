@@ -118,62 +118,74 @@ __device__ __noinline__ void bfly_k_any(const float2* sp, float2* dp, int R, int
     }
 }
 
-// lds_fft (pf_fft.h) with each thread stepping through its butterflies by the
-// block size: (transform f, butterfly j) and k = j mod Ns advance by adds and
-// compares (nb is a multiple of Ns, so wrapping j past nb keeps k), and the
-// twiddles are read from LDS.  Returns the buffer holding the result; ends
-// with a barrier.
+// One stage of lds_fft (pf_fft.h) for radix R (0: any radix, generic DFT),
+// with each thread stepping through its butterflies by the block size:
+// (transform f, butterfly j) and k = j mod Ns advance by adds and compares
+// (nb is a multiple of Ns, so wrapping j past nb keeps k).
+template <int R, int DIR>
+__device__ __forceinline__ void rt_stage(const float2* a, float2* b, int Rr, int Ns, int n,
+                                         const float2* tw, int nfft, int fs, int es) {
+    const int T = blockDim.x, t = threadIdx.x;
+    const int RR = R > 0 ? R : Rr;
+    const int nb = n / RR, step = n / (Ns * RR), total = nfft * nb;
+    int f, j, df, dj;
+    if (es == 1) {   // u = f nb + j
+        f = t / nb;
+        j = t - f * nb;
+        df = T / nb;
+        dj = T - df * nb;
+    } else {         // u = j nfft + f
+        j = t / nfft;
+        f = t - j * nfft;
+        dj = T / nfft;
+        df = T - dj * nfft;
+    }
+    int k = j % Ns;
+    const int dk = dj % Ns;
+    for (int u = t; u < total; u += T) {
+        const float2* sp = a + f * fs;
+        float2* dp = b + f * fs;
+        if constexpr (R > 0) bfly_k<R, DIR>(sp, dp, j, k, Ns, nb, step, es, tw);
+        else bfly_k_any<DIR>(sp, dp, RR, j, k, Ns, nb, step, es, tw);
+        k += dk;
+        if (k >= Ns) k -= Ns;
+        if (es == 1) {
+            j += dj;
+            f += df;
+            if (j >= nb) {
+                j -= nb;
+                ++f;
+            }
+        } else {
+            f += df;
+            j += dj;
+            if (f >= nfft) {
+                f -= nfft;
+                ++j;
+                if (++k >= Ns) k -= Ns;
+            }
+        }
+    }
+}
+
+// lds_fft (pf_fft.h) over nfft transforms in LDS with the radix of each stage
+// dispatched once per stage; twiddles are read from LDS.  Returns the buffer
+// holding the result; ends with a barrier.
 template <int DIR>
 __device__ float2* lds_fft_rt(float2* a, float2* b, const Plan1D& p, const float2* tw, int nfft,
                               int fs, int es) {
-    const int n = p.n, T = blockDim.x, t = threadIdx.x;
+    const int n = p.n;
     int Ns = 1;
     for (int s = 0; s < p.nstages; ++s) {
         const int R = p.radix[s];
-        const int nb = n / R, step = n / (Ns * R), total = nfft * nb;
-        int f, j, df, dj;
-        if (es == 1) {   // u = f nb + j
-            f = t / nb;
-            j = t - f * nb;
-            df = T / nb;
-            dj = T - df * nb;
-        } else {         // u = j nfft + f
-            j = t / nfft;
-            f = t - j * nfft;
-            dj = T / nfft;
-            df = T - dj * nfft;
-        }
-        int k = j % Ns;
-        const int dk = dj % Ns;
-        for (int u = t; u < total; u += T) {
-            const float2* sp = a + f * fs;
-            float2* dp = b + f * fs;
-            switch (R) {
-                case 2: bfly_k<2, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
-                case 3: bfly_k<3, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
-                case 4: bfly_k<4, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
-                case 5: bfly_k<5, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
-                case 8: bfly_k<8, DIR>(sp, dp, j, k, Ns, nb, step, es, tw); break;
-                default: bfly_k_any<DIR>(sp, dp, R, j, k, Ns, nb, step, es, tw); break;
-            }
-            k += dk;
-            if (k >= Ns) k -= Ns;
-            if (es == 1) {
-                j += dj;
-                f += df;
-                if (j >= nb) {
-                    j -= nb;
-                    ++f;
-                }
-            } else {
-                f += df;
-                j += dj;
-                if (f >= nfft) {
-                    f -= nfft;
-                    ++j;
-                    if (++k >= Ns) k -= Ns;
-                }
-            }
+        switch (R) {
+            case 2: rt_stage<2, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
+            case 3: rt_stage<3, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
+            case 4: rt_stage<4, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
+            case 5: rt_stage<5, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
+            case 7: rt_stage<7, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
+            case 8: rt_stage<8, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
+            default: rt_stage<0, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
         }
         __syncthreads();
         float2* tmp = a;
@@ -183,6 +195,26 @@ __device__ float2* lds_fft_rt(float2* a, float2* b, const Plan1D& p, const float
     }
     return a;
 }
+
+// Row-major stepping over an (rows x cols) index space by the block size:
+// (r, c) of u = threadIdx.x, advanced by adds and compares.
+struct Step2 {
+    int r, c, dr, dc, cols;
+    __device__ __forceinline__ Step2(int cols_) : cols(cols_) {
+        r = (int)threadIdx.x / cols;
+        c = (int)threadIdx.x - r * cols;
+        dr = (int)blockDim.x / cols;
+        dc = (int)blockDim.x - dr * cols;
+    }
+    __device__ __forceinline__ void next() {
+        r += dr;
+        c += dc;
+        if (c >= cols) {
+            c -= cols;
+            ++r;
+        }
+    }
+};
 
 // Copy a twiddle table into LDS.
 __device__ __forceinline__ void stage_tw(float2* dst, const float2* __restrict__ src, int n) {
@@ -226,8 +258,9 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
     const int rowM = g.ny * g.M;
     for (int gi = 0; gi < ngrid; ++gi) {
         // rows: (v(2m), v(2m+1)) packed as complex, row stride M
-        for (int u = threadIdx.x; u < rowM; u += blockDim.x) {
-            const int y = u / g.M, m = u - y * g.M;
+        Step2 it(g.M);
+        for (int u = threadIdx.x; u < rowM; u += blockDim.x, it.next()) {
+            const int y = it.r, m = it.c;
             const int sy = scoord(y, r, g.ny);
             float v2[2] = {0.f, 0.f};
             if (sz * sz + sy * sy <= rg2) {
@@ -247,8 +280,9 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
         float2* Z = lds_fft_rt<-1>(A, B, px, twx, g.ny, g.M, 1);
         float2* Xp = Z == A ? B : A;
         // R2C post-processing, rows of nxh: X[k] = E[k] + w^k O[k]
-        for (int u = threadIdx.x; u < g.plane; u += blockDim.x) {
-            const int y = u / g.nxh, k = u - y * g.nxh;
+        Step2 ip(g.nxh);
+        for (int u = threadIdx.x; u < g.plane; u += blockDim.x, ip.next()) {
+            const int y = ip.r, k = ip.c;
             const float2* z = Z + y * g.M;
             float2 X;
             if (k == 0 || k == g.M) {
@@ -293,8 +327,9 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
     // a forward grid's column tile; planes beyond the support were not
     // written by the forward stage: they are zero
     auto load = [&](const float2* src, float2* d) {
-        for (int u = threadIdx.x; u < n; u += blockDim.x) {
-            const int z = u / nc, c = u - z * nc;
+        Step2 it(nc);
+        for (int u = threadIdx.x; u < n; u += blockDim.x, it.next()) {
+            const int z = it.r, c = it.c;
             const int sz = z <= g.nz / 2 ? z : z - g.nz;
             d[z * kRtTq + c] = (sz >= -zr && sz <= zr) ? src[(int64_t)z * g.plane + q0 + c]
                                                        : c2(0.f, 0.f);
@@ -302,16 +337,18 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
         __syncthreads();
     };
     auto store = [&](const float2* s, float2* d) {
-        for (int u = threadIdx.x; u < n; u += blockDim.x) {
-            const int z = u / nc, c = u - z * nc;
+        Step2 it(nc);
+        for (int u = threadIdx.x; u < n; u += blockDim.x, it.next()) {
+            const int z = it.r, c = it.c;
             d[(int64_t)z * g.plane + q0 + c] = s[z * kRtTq + c];
         }
     };
     // in place: s = conj(s) * t (and, with t2, d = conj(s) * t2 first); t, t2
     // are the target spectra at this tile's columns (row stride plane)
     auto mul = [&](float2* s, const float2* t, float2* d, const float2* t2) {
-        for (int u = threadIdx.x; u < n; u += blockDim.x) {
-            const int z = u / nc, c = u - z * nc;
+        Step2 it(nc);
+        for (int u = threadIdx.x; u < n; u += blockDim.x, it.next()) {
+            const int z = it.r, c = it.c;
             const int i = z * kRtTq + c;
             const int64_t gi = (int64_t)z * g.plane + c;
             const float2 x = cconj(s[i]);
@@ -394,8 +431,9 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
             float2* D = Y == A ? B : A;
             // C2R pre-processing, rows of M: z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
             const int rowM = g.ny * g.M;
-            for (int u = threadIdx.x; u < rowM; u += blockDim.x) {
-                const int y = u / g.M, k = u - y * g.M;
+            Step2 it(g.M);
+            for (int u = threadIdx.x; u < rowM; u += blockDim.x, it.next()) {
+                const int y = it.r, k = it.c;
                 const float2* xr = Y + y * g.nxh;
                 float2 xk = xr[k], xm = xr[g.M - k];
                 if (k == 0) {
