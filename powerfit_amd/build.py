@@ -1,4 +1,5 @@
-"""Build libpowerfit_hip.so (gfx950) and the host-side FFT check library in-tree.
+"""Build libpowerfit_hip.so (gfx950) and the host-side FFT check library in-tree
+(one compiler process per HIP source, in parallel, then one link).
 
     python -m powerfit_amd.build
 
@@ -32,13 +33,30 @@ def _run(cmd):
 
 
 def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    """Compile each HIP source to an object in parallel, then link the library."""
+    objdir = os.path.join(HERE, "_objs")
+    os.makedirs(objdir, exist_ok=True)
+    procs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-c", *COMMON, *NO_PACKED_FP32,
+               os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        procs.append((cmd, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                 stderr=subprocess.PIPE, text=True)))
+    objs = []
+    failed = None
+    for cmd, obj, proc in procs:
+        out, err = proc.communicate()
+        if proc.returncode != 0 and failed is None:
+            sys.stderr.write(out + err)
+            failed = cmd
+        objs.append(obj)
+    if failed is not None:
+        raise RuntimeError("build step failed: " + " ".join(failed))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", *COMMON, *NO_PACKED_FP32,
-           *srcs, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd))
-    _run(cmd)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
     # host build of the FFT stage code (CPU checks of the device butterflies)
     tmp = HOST_LIB + ".tmp"

@@ -1,0 +1,179 @@
+// LDS FFT passes with the transform length fixed at compile time, for the
+// run-time planned fused pipeline (pf_planes.hip).
+//
+// A pass runs nfft transforms of length L held in LDS as Stockham stages that
+// ping-pong between two buffers, one workgroup barrier per stage.  The radix
+// plan is chosen at compile time from radices 2..16 (composite ones run in
+// registers as nested small DFTs), fewest stages first, so most 2,3,5,7-
+// smooth lengths up to 256 need two stages.  Every index and twiddle step is
+// a compile-time constant; each thread takes butterflies u = threadIdx.x +
+// k blockDim.x of the pass.
+//
+// Layouts: ROWS — transform f at a[f * st + e] (element stride 1), the
+// threads of a row are consecutive; COLS — transform f at a[f + e * st]
+// (element stride st), consecutive threads take consecutive transforms at the
+// same element (conflict-free LDS columns).
+//
+// ct_dispatch<DIR, COLS>(L, ...) switches over the lengths listed below and
+// returns false for any other length (the caller then runs the run-time
+// planned pass).
+#pragma once
+#include "pf_group_fft.h"
+
+namespace pf {
+
+struct CtPlan {
+    int ns = 0;       // stages (-1: no plan)
+    int maxr = 0;     // largest radix
+    int r[6] = {0, 0, 0, 0, 0, 0};
+};
+
+constexpr int kCtRadices[13] = {16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+
+// Fewest stages, then the smallest largest radix (balanced stages).
+constexpr CtPlan ct_plan(int L) {
+    CtPlan best;
+    if (L == 1) return best;
+    best.ns = -1;
+    for (int i = 0; i < 13; ++i) {
+        const int R = kCtRadices[i];
+        if (L % R != 0) continue;
+        const CtPlan sub = ct_plan(L / R);
+        if (sub.ns < 0 || sub.ns >= 5) continue;
+        const int ns = sub.ns + 1;
+        const int maxr = sub.maxr > R ? sub.maxr : R;
+        if (best.ns < 0 || ns < best.ns || (ns == best.ns && maxr < best.maxr)) {
+            best.ns = ns;
+            best.maxr = maxr;
+            best.r[0] = R;
+            for (int s = 0; s < sub.ns; ++s) best.r[s + 1] = sub.r[s];
+        }
+    }
+    return best;
+}
+
+constexpr int ct_span(const CtPlan& p, int s) {
+    int ns = 1;
+    for (int i = 0; i < s; ++i) ns *= p.r[i];
+    return ns;
+}
+
+// Butterfly j of stage S of the length-L plan: element e of the transform at
+// sp[e * es] (input) and dp[e * es] (output).  tw[m] = exp(-2 pi i m / L).
+template <int L, int DIR, int S>
+PF_HD void ct_bfly(const float2* sp, float2* dp, int j, int es, const float2* tw) {
+    constexpr CtPlan P = ct_plan(L);
+    constexpr int R = P.r[S];
+    constexpr int Ns = ct_span(P, S);
+    constexpr int NB = L / R;
+    constexpr int STEP = L / (Ns * R);
+    const int k = Ns == 1 ? 0 : j % Ns;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = sp[(j + r * NB) * es];
+    if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<DIR>(tw, r * k * STEP));
+    }
+    dft_any<R, DIR>(v);
+    const int od = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dp[(od + r * Ns) * es] = v[r];
+}
+
+// Sequential run of the whole plan on the host (CPU checks); returns the
+// buffer holding the result.
+template <int L, int DIR, int S = 0>
+inline float2* ct_host_run(float2* a, float2* b, const float2* tw) {
+    constexpr CtPlan P = ct_plan(L);
+    if constexpr (S >= P.ns) {
+        return a;
+    } else {
+        for (int j = 0; j < L / P.r[S]; ++j) ct_bfly<L, DIR, S>(a, b, j, 1, tw);
+        return ct_host_run<L, DIR, S + 1>(b, a, tw);
+    }
+}
+
+#if defined(__HIP__)
+template <int L, int DIR, bool COLS, int S>
+__device__ __forceinline__ void ct_stage(const float2* a, float2* b, int nfft, int st,
+                                         const float2* tw) {
+    constexpr CtPlan P = ct_plan(L);
+    constexpr int NB = L / P.r[S];
+    const int total = nfft * NB;
+    if constexpr (!COLS) {
+        for (int u = threadIdx.x; u < total; u += blockDim.x) {
+            const int f = u / NB, j = u - f * NB;
+            ct_bfly<L, DIR, S>(a + f * st, b + f * st, j, 1, tw);
+        }
+    } else {
+        // u = j nfft + f, stepped without divisions
+        int j = (int)threadIdx.x / nfft;
+        int f = (int)threadIdx.x - j * nfft;
+        const int dj = (int)blockDim.x / nfft, df = (int)blockDim.x - dj * nfft;
+        for (int u = threadIdx.x; u < total; u += blockDim.x) {
+            ct_bfly<L, DIR, S>(a + f, b + f, j, st, tw);
+            j += dj;
+            f += df;
+            if (f >= nfft) {
+                f -= nfft;
+                ++j;
+            }
+        }
+    }
+}
+
+template <int L, int DIR, bool COLS, int S = 0>
+__device__ __forceinline__ float2* ct_run(float2* a, float2* b, int nfft, int st,
+                                          const float2* tw) {
+    constexpr CtPlan P = ct_plan(L);
+    static_assert(P.ns >= 0, "no compile-time plan for this length");
+    if constexpr (S >= P.ns) {
+        return a;
+    } else {
+        ct_stage<L, DIR, COLS, S>(a, b, nfft, st, tw);
+        __syncthreads();
+        return ct_run<L, DIR, COLS, S + 1>(b, a, nfft, st, tw);
+    }
+}
+#endif
+
+// Lengths with a compiled pass: rows (half the x length of a plane) and
+// columns / z (the y and z lengths).  Every 2,3,5,7-smooth length in
+// [8, 128] and [16, 256] respectively.
+#define PF_CT_ROW_LENGTHS(X) \
+    X(8) X(9) X(10) X(12) X(14) X(15) X(16) X(18) X(20) X(21) X(24) X(25) X(27) X(28) X(30) \
+    X(32) X(35) X(36) X(40) X(42) X(45) X(48) X(49) X(50) X(54) X(56) X(60) X(63) X(64) X(70) \
+    X(72) X(75) X(80) X(81) X(84) X(90) X(96) X(98) X(100) X(105) X(108) X(112) X(120) X(125) \
+    X(126) X(128)
+#define PF_CT_COL_LENGTHS(X) \
+    X(16) X(18) X(20) X(21) X(24) X(25) X(27) X(28) X(30) X(32) X(35) X(36) X(40) X(42) X(45) \
+    X(48) X(49) X(50) X(54) X(56) X(60) X(63) X(64) X(70) X(72) X(75) X(80) X(81) X(84) X(90) \
+    X(96) X(98) X(100) X(105) X(108) X(112) X(120) X(125) X(126) X(128) X(135) X(140) X(144) \
+    X(147) X(150) X(160) X(162) X(168) X(175) X(180) X(189) X(192) X(196) X(200) X(210) X(216) \
+    X(224) X(225) X(240) X(243) X(245) X(250) X(252) X(256)
+
+#if defined(__HIP__)
+// Run a compiled pass for length L if there is one: *out receives the buffer
+// holding the result.  ROWS lengths for !COLS, column lengths for COLS.
+template <int DIR, bool COLS>
+__device__ __forceinline__ bool ct_dispatch(int L, float2* a, float2* b, int nfft, int st,
+                                            const float2* tw, float2** out) {
+#define PF_CT_CASE(n) \
+    case n: *out = ct_run<n, DIR, COLS>(a, b, nfft, st, tw); return true;
+    if constexpr (COLS) {
+        switch (L) {
+            PF_CT_COL_LENGTHS(PF_CT_CASE)
+            default: return false;
+        }
+    } else {
+        switch (L) {
+            PF_CT_ROW_LENGTHS(PF_CT_CASE)
+            default: return false;
+        }
+    }
+#undef PF_CT_CASE
+}
+#endif
+
+}  // namespace pf

@@ -136,3 +136,42 @@ extern "C" int pf_host_group_fft_pad(int n, int t, int pad, int to_ex, int dir, 
     return -1;
 }
 
+
+// Host run of the compile-time-length LDS passes (pf_ct_fft.h) for every
+// length they are compiled for; returns the number of stages, -1 for a length
+// without a compiled pass.
+#include "pf_ct_fft.h"
+
+template <int L>
+static int ct_fft_host(int dir, const float* in, float* out) {
+    std::vector<float2> tw(L), a(L), b(L);
+    for (int m = 0; m < L; ++m) {
+        const double ang = -2.0 * M_PI * (double)m / (double)L;
+        tw[m] = pf::c2((float)std::cos(ang), (float)std::sin(ang));
+        a[m] = pf::c2(in[2 * m], in[2 * m + 1]);
+    }
+    const float2* r = dir < 0 ? pf::ct_host_run<L, -1>(a.data(), b.data(), tw.data())
+                              : pf::ct_host_run<L, 1>(a.data(), b.data(), tw.data());
+    for (int m = 0; m < L; ++m) {
+        out[2 * m] = r[m].x;
+        out[2 * m + 1] = r[m].y;
+    }
+    return pf::ct_plan(L).ns;
+}
+
+extern "C" int pf_host_ct_fft(int n, int dir, const float* in, float* out) {
+    switch (n) {
+#define PF_HOST_CASE(L) case L: return ct_fft_host<L>(dir, in, out);
+        PF_CT_COL_LENGTHS(PF_HOST_CASE)
+        PF_HOST_CASE(8) PF_HOST_CASE(9) PF_HOST_CASE(10) PF_HOST_CASE(12) PF_HOST_CASE(14)
+        PF_HOST_CASE(15)
+#undef PF_HOST_CASE
+        default: return -1;
+    }
+}
+
+extern "C" int pf_host_ct_plan(int n, int* radix, int max) {
+    const pf::CtPlan p = pf::ct_plan(n);
+    for (int s = 0; s < p.ns && s < max; ++s) radix[s] = p.r[s];
+    return p.ns;
+}

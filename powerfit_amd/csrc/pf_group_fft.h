@@ -184,8 +184,11 @@ template <int R, int DIR>
 PF_HD void dft_any(float2* v) {
     if constexpr (R == 16) dft16<DIR>(v);
     else if constexpr (R == 6) dft_ab<2, 3, DIR>(v);
+    else if constexpr (R == 9) dft_ab<3, 3, DIR>(v);
     else if constexpr (R == 10) dft_ab<2, 5, DIR>(v);
     else if constexpr (R == 12) dft_ab<4, 3, DIR>(v);
+    else if constexpr (R == 14) dft_ab<2, 7, DIR>(v);
+    else if constexpr (R == 15) dft_ab<3, 5, DIR>(v);
     else if constexpr (R == 20) dft_ab<4, 5, DIR>(v);
     else if constexpr (R == 24) dft_ab<8, 3, DIR>(v);
     else dft_fixed<R, DIR>(v);

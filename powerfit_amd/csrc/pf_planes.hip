@@ -24,6 +24,7 @@
 // in LDS; at most kRtVox voxels of a plane per thread of the LCC stage.
 #include <algorithm>
 
+#include "pf_ct_fft.h"
 #include "pf_internal.h"
 #include "pf_rotate.h"
 
@@ -196,6 +197,17 @@ __device__ float2* lds_fft_rt(float2* a, float2* b, const Plan1D& p, const float
     return a;
 }
 
+// Transforms of a plane or column tile: the compiled pass for the length
+// when there is one (pf_ct_fft.h), else the run-time planned one.  COLS:
+// transform f at a[f + e st]; otherwise at a[f st + e].
+template <int DIR, bool COLS>
+__device__ __forceinline__ float2* tile_fft(float2* a, float2* b, const Plan1D& p,
+                                            const float2* tw, int nfft, int st) {
+    float2* out;
+    if (ct_dispatch<DIR, COLS>(p.n, a, b, nfft, st, tw, &out)) return out;
+    return lds_fft_rt<DIR>(a, b, p, tw, nfft, COLS ? 1 : st, COLS ? st : 1);
+}
+
 // Row-major stepping over an (rows x cols) index space by the block size:
 // (r, c) of u = threadIdx.x, advanced by adds and compares.
 struct Step2 {
@@ -277,7 +289,7 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
             A[u] = c2(v2[0], v2[1]);
         }
         __syncthreads();
-        float2* Z = lds_fft_rt<-1>(A, B, px, twx, g.ny, g.M, 1);
+        float2* Z = tile_fft<-1, false>(A, B, px, twx, g.ny, g.M);
         float2* Xp = Z == A ? B : A;
         // R2C post-processing, rows of nxh: X[k] = E[k] + w^k O[k]
         Step2 ip(g.nxh);
@@ -298,7 +310,7 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
         }
         __syncthreads();
         // y FFT of the nxh columns (element stride nxh)
-        const float2* Y = lds_fft_rt<-1>(Xp, Z, py, twy, g.nxh, 1, g.nxh);
+        const float2* Y = tile_fft<-1, true>(Xp, Z, py, twy, g.nxh, g.nxh);
         float2* o = dst + (int64_t)gi * g.H;
         for (int u = threadIdx.x; u < g.plane; u += blockDim.x) o[u] = Y[u];
         __syncthreads();
@@ -357,37 +369,29 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
         }
         __syncthreads();
     };
+    // Per rotation three outputs: G = IFFT(conj(FP) F1), A = IFFT(conj(FW) F1),
+    // B = IFFT(conj(FW) F2) (FW^2 for weighted masks).  With a 0/1 mask FW
+    // is transformed once and its F2 product parked in S2 for B.  One
+    // forward and one inverse call site keep the kernel's code small.
     for (int b = b0; b < b1; ++b) {
         float2* base = X + (int64_t)b * 3 * g.H;
-        // G = IFFT(conj(FP) F1)
-        load(base, S0);
-        float2* Z = lds_fft_rt<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
-        mul(Z, T1, nullptr, nullptr);
-        const float2* Zi = lds_fft_rt<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
-        store(Zi, base);
-        __syncthreads();
-        // A = IFFT(conj(FW) F1); B = IFFT(conj(FW) F2) or IFFT(conj(FW^2) F2)
-        load(base + g.H, S0);
-        Z = lds_fft_rt<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
-        if (bin) {
-            mul(Z, T1, S2, T2);
-            Zi = lds_fft_rt<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
-            store(Zi, base + g.H);
+        for (int o = 0; o < 3; ++o) {
+            float2* Z;
+            float2* other;
+            if (bin && o == 2) {
+                Z = S2;
+                other = S0;
+            } else {
+                load(base + (int64_t)(bin ? (o > 0 ? 1 : 0) : o) * g.H, S0);
+                Z = tile_fft<-1, true>(S0, S1, pz, twz, nc, kRtTq);
+                other = Z == S0 ? S1 : S0;
+                const bool both = bin && o == 1;
+                mul(Z, o == 2 ? T2 : T1, both ? S2 : nullptr, both ? T2 : nullptr);
+            }
+            const float2* Zi = tile_fft<1, true>(Z, other, pz, twz, nc, kRtTq);
+            store(Zi, base + (int64_t)o * g.H);
             __syncthreads();
-            Zi = lds_fft_rt<1>(S2, S0, pz, twz, nc, 1, kRtTq);
-            store(Zi, base + 2 * g.H);
-        } else {
-            mul(Z, T1, nullptr, nullptr);
-            Zi = lds_fft_rt<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
-            store(Zi, base + g.H);
-            __syncthreads();
-            load(base + 2 * g.H, S0);
-            Z = lds_fft_rt<-1>(S0, S1, pz, twz, nc, 1, kRtTq);
-            mul(Z, T2, nullptr, nullptr);
-            Zi = lds_fft_rt<1>(Z, Z == S0 ? S1 : S0, pz, twz, nc, 1, kRtTq);
-            store(Zi, base + 2 * g.H);
         }
-        __syncthreads();
     }
 }
 
@@ -427,7 +431,7 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
             for (int u = threadIdx.x; u < g.plane; u += blockDim.x) A[u] = src[u];
             __syncthreads();
             // inverse y over the nxh columns
-            float2* Y = lds_fft_rt<1>(A, B, py, twy, g.nxh, 1, g.nxh);
+            float2* Y = tile_fft<1, true>(A, B, py, twy, g.nxh, g.nxh);
             float2* D = Y == A ? B : A;
             // C2R pre-processing, rows of M: z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
             const int rowM = g.ny * g.M;
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
                 D[u] = cadd(cadd(xk, xmc), mul_si<1>(d));
             }
             __syncthreads();
-            const float2* Z = lds_fft_rt<1>(D, Y, px, twx, g.ny, g.M, 1);
+            const float2* Z = tile_fft<1, false>(D, Y, px, twx, g.ny, g.M);
             const float* zr = reinterpret_cast<const float*>(Z);   // row y: 2 M reals
 #pragma unroll
             for (int k = 0; k < kRtVox; ++k) {

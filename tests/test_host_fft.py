@@ -78,3 +78,35 @@ def test_group_fft_padded_exchanges_match_numpy(host, n, t, pad):
             slots = f(n, t, pad, to_ex, d, x.ctypes.data_as(FP), out.ctypes.data_as(FP))
             assert slots == n + (n - 1) // pad
             assert np.abs(out - ref).max() / np.abs(ref).max() < 1e-6
+
+
+def _smooth(n):
+    for p in (2, 3, 5, 7):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+CT_LENGTHS = [n for n in range(8, 257) if _smooth(n)]
+
+
+@pytest.mark.parametrize("n", CT_LENGTHS)
+def test_compile_time_length_passes_match_numpy(host, n):
+    """The compile-time-length LDS passes of the run-time planned pipeline
+    (every 2,3,5,7-smooth length from 8 to 256), run stage by stage on the
+    host, equal numpy's transform in both directions; their plans use radices
+    up to 16: one stage up to 16 points, two wherever a two-factor split
+    into such radices exists, else three."""
+    rng = np.random.default_rng(n + 7)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    buf = (ctypes.c_int * 8)()
+    ns = host.pf_host_ct_plan(n, buf, 8)
+    radices = list(buf[:ns])
+    assert int(np.prod(radices)) == n and max(radices) <= 16
+    two = any(n % a == 0 and n // a <= 16 for a in range(2, 17))
+    assert ns == (1 if n <= 16 else 2 if two else 3), radices
+    for d in (-1, 1):
+        out = np.zeros(n, np.complex64)
+        assert host.pf_host_ct_fft(n, d, x.ctypes.data_as(FP), out.ctypes.data_as(FP)) == ns
+        ref = np.fft.fft(x.astype(np.complex128)) if d < 0 else np.fft.ifft(x) * n
+        assert np.abs(out - ref).max() / np.abs(ref).max() < 2e-6, (n, d, radices)
