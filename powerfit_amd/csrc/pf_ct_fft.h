@@ -176,4 +176,201 @@ __device__ __forceinline__ bool ct_dispatch(int L, float2* a, float2* b, int nff
 }
 #endif
 
+
+// ---------------------------------------------------------------------------
+// Register FFTs of compile-time length L by T threads with stage-by-stage
+// butterfly counts (no need for every radix to divide the elements a thread
+// holds).  Stage s of radix R has L / R butterflies; thread t runs
+// j = t + T b for b < ceil(L / (R T)) (the last ones partly idle) with its R
+// inputs in registers v[b R + r] = x[j + r L / R]; the outputs go to an LDS
+// exchange buffer at their natural positions and the next stage reads its
+// own inputs back.
+//
+// Register patterns: a transform enters in the input pattern of its first
+// radix and leaves in the output pattern of its last, both of the form
+// slot b R + r <-> element j + r L / R.  The inverse runs the stages in
+// reverse order (REV), so the forward's output pattern is the inverse's
+// input pattern and its output pattern the forward's input pattern: a
+// forward transform, a pointwise product and the inverse need no exchange
+// between them.
+struct FlexPlan {
+    int ns = -1;
+    int T = 0;
+    int E = 0;        // register slots per thread (largest stage)
+    int cost = 0;
+    int r[4] = {0, 0, 0, 0};
+};
+
+constexpr int flex_slots(int L, int R, int T) { return ((L / R + T - 1) / T) * R; }
+
+// Cost of an ordered radix list over T threads: thread-slots of work summed
+// over stages plus an exchange charge per stage boundary.
+constexpr void flex_consider(FlexPlan& best, const int* r, int ns, int L, int emax, int tmax,
+                             bool pow2) {
+    for (int T = 1; T <= tmax; ++T) {
+        if (pow2 && (T & (T - 1)) != 0) continue;
+        int E = 0, work = 0;
+        for (int s = 0; s < ns; ++s) {
+            const int sl = flex_slots(L, r[s], T);
+            E = sl > E ? sl : E;
+            work += sl;
+        }
+        if (E > emax) continue;
+        const int cost = T * work + (ns - 1) * L;
+        if (best.ns < 0 || cost < best.cost || (cost == best.cost && T > best.T)) {
+            best.ns = ns;
+            best.T = T;
+            best.E = E;
+            best.cost = cost;
+            for (int s = 0; s < 4; ++s) best.r[s] = s < ns ? r[s] : 0;
+        }
+    }
+}
+
+constexpr void flex_search(FlexPlan& best, int* r, int depth, int rem, int L, int emax,
+                           int tmax, bool pow2) {
+    if (rem == 1) {
+        if (depth > 0) flex_consider(best, r, depth, L, emax, tmax, pow2);
+        return;
+    }
+    if (depth == 3) return;
+    for (int i = 0; i < 13; ++i) {
+        const int R = kCtRadices[i];
+        if (rem % R != 0) continue;
+        r[depth] = R;
+        flex_search(best, r, depth + 1, rem / R, L, emax, tmax, pow2);
+    }
+}
+
+// Best ordered plan of at most three stages and the thread count for L
+// points with at most emax register slots and tmax threads per transform
+// (pow2: thread counts that are powers of two only, so blocks of such
+// transforms are whole wavefronts).
+constexpr FlexPlan flex_plan(int L, int emax, int tmax, bool pow2) {
+    FlexPlan best;
+    int r[4] = {0, 0, 0, 0};
+    flex_search(best, r, 0, L, L, emax, tmax, pow2);
+    return best;
+}
+
+template <int L, int EMAX, int TMAX, bool REV, bool POW2 = false>
+struct Flex {
+    static constexpr FlexPlan F = flex_plan(L, EMAX, TMAX, POW2);
+    static_assert(F.ns >= 1, "no register plan for this length");
+    static constexpr int T = F.T, E = F.E, NS = F.ns;
+    static constexpr int radix(int s) { return REV ? F.r[NS - 1 - s] : F.r[s]; }
+    static constexpr int span(int s) {
+        int ns = 1;
+        for (int i = 0; i < s; ++i) ns *= radix(i);
+        return ns;
+    }
+    static constexpr int bpt(int s) { return (L / radix(s) + T - 1) / T; }
+    // element held in slot b R + r of the input (first stage) / output (last
+    // stage) pattern, or -1 for an idle slot
+    static PF_HD int in_pos(int t, int slot) {
+        constexpr int R = radix(0), NB = L / R;
+        const int b = slot / R, r = slot - b * R, j = t + T * b;
+        return (slot < bpt(0) * R && j < NB) ? j + r * NB : -1;
+    }
+    static PF_HD int out_pos(int t, int slot) {
+        constexpr int R = radix(NS - 1), NB = L / R;
+        const int b = slot / R, r = slot - b * R, j = t + T * b;
+        return (slot < bpt(NS - 1) * R && j < NB) ? j + r * NB : -1;
+    }
+    static constexpr int in_slots() { return bpt(0) * radix(0); }
+    static constexpr int out_slots() { return bpt(NS - 1) * radix(NS - 1); }
+};
+
+template <int L, int EMAX, int TMAX, bool REV, int DIR, int S, bool POW2 = false>
+struct FlexStage {
+    using FX = Flex<L, EMAX, TMAX, REV, POW2>;
+    static constexpr int T = FX::T, E = FX::E;
+    static constexpr bool kLast = S + 1 >= FX::NS;
+    static constexpr int R = FX::radix(S), NB = L / R, Ns = FX::span(S);
+    static constexpr int STEP = L / (Ns * R), BPT = FX::bpt(S);
+
+    // this stage's butterflies in registers (twiddles, then the R-point DFT)
+    static PF_HD void bfly(float2 (&v)[E], int t, const float2* __restrict__ tw) {
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            const int j = t + T * b;
+            if (BPT * T > NB && j >= NB) continue;
+            const int k = Ns == 1 ? 0 : j % Ns;
+            float2 u[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) u[r] = v[b * R + r];
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) u[r] = cmul(u[r], twid<DIR>(tw, r * k * STEP));
+            }
+            dft_any<R, DIR>(u);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b * R + r] = u[r];
+        }
+    }
+
+    // outputs to the exchange buffer at their natural positions
+    static PF_HD void write(const float2 (&v)[E], float2* ex, int es, int t) {
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            const int j = t + T * b;
+            if (BPT * T > NB && j >= NB) continue;
+            const int k = Ns == 1 ? 0 : j % Ns;
+            const int od = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) ex[(od + r * Ns) * es] = v[b * R + r];
+        }
+    }
+
+    // the next stage's inputs back from the exchange buffer
+    static PF_HD void read_next(float2 (&v)[E], const float2* ex, int es, int t) {
+        if constexpr (!kLast) {
+            constexpr int R2 = FX::radix(S + 1), NB2 = L / R2, BPT2 = FX::bpt(S + 1);
+#pragma unroll
+            for (int b = 0; b < BPT2; ++b) {
+                const int j = t + T * b;
+                if (BPT2 * T > NB2 && j >= NB2) continue;
+#pragma unroll
+                for (int r = 0; r < R2; ++r) v[b * R2 + r] = ex[(j + r * NB2) * es];
+            }
+        }
+    }
+
+#if defined(__HIP__)
+    static __device__ __forceinline__ void run(float2 (&v)[E], float2* ex, int es, int t,
+                                               const float2* __restrict__ tw, bool active) {
+        if (active) bfly(v, t, tw);
+        if constexpr (!kLast) {
+            if (active) write(v, ex, es, t);
+            __syncthreads();
+            if (active) read_next(v, ex, es, t);
+            __syncthreads();
+            FlexStage<L, EMAX, TMAX, REV, DIR, S + 1, POW2>::run(v, ex, es, t, tw, active);
+        }
+    }
+#endif
+
+    // all T threads of one transform in sequence (host checks)
+    static void emulate(float2 (*v)[E], float2* ex, const float2* tw) {
+        for (int t = 0; t < T; ++t) bfly(v[t], t, tw);
+        if constexpr (!kLast) {
+            for (int t = 0; t < T; ++t) write(v[t], ex, 1, t);
+            for (int t = 0; t < T; ++t) read_next(v[t], ex, 1, t);
+            FlexStage<L, EMAX, TMAX, REV, DIR, S + 1, POW2>::emulate(v, ex, tw);
+        }
+    }
+};
+
+#if defined(__HIP__)
+// L-point transform of the T threads' registers (input pattern in, output
+// pattern out), exchanging through ex[e * es].  Contains workgroup barriers:
+// every thread of the block calls it (active = false for idle transforms).
+template <int L, int EMAX, int TMAX, bool REV, int DIR, bool POW2, int E>
+__device__ __forceinline__ void flex_fft(float2 (&v)[E], float2* ex, int es, int t,
+                                         const float2* __restrict__ tw, bool active) {
+    static_assert(E == Flex<L, EMAX, TMAX, REV, POW2>::E, "register array of the plan's size");
+    FlexStage<L, EMAX, TMAX, REV, DIR, 0, POW2>::run(v, ex, es, t, tw, active);
+}
+#endif
+
 }  // namespace pf

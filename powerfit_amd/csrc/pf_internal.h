@@ -139,6 +139,10 @@ int prepare_planes_rt();
 // f's plans and twiddles and a's buffers and options (a.n, a.occ and
 // a.skip_planes are not used: every plane is transformed).
 int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t s);
+// Its z stage with the z length compiled in (pf_zct.hip): lengths with a
+// compiled register transform; zr as for the LDS-planned z stage.
+bool zcorr_ct_supported(int nz);
+int launch_zcorr_ct(const Fft3D& f, const FusedArgs& a, int zr, hipStream_t s);
 
 // 1 bit per 8^3 brick of an n^3 interleaved grid (n % 8 == 0): ((n/8)^3 + 31) / 32 words.
 int launch_brick_occupancy(const float2* tm, int n, uint32_t* occ, hipStream_t s);

@@ -267,6 +267,10 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
     float2* dst = out + (int64_t)b * 3 * g.H + (int64_t)iz * g.plane;
+    // The mask values gathered with P are parked in this plane's region of
+    // grid 2 (written last, by this block only, after every read of it; each
+    // thread reads back only what it wrote) for the W (and W^2) grids.
+    float2* const wst = dst + 2 * g.H;
     const int rowM = g.ny * g.M;
     for (int gi = 0; gi < ngrid; ++gi) {
         // rows: (v(2m), v(2m+1)) packed as complex, row stride M
@@ -274,19 +278,32 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
         for (int u = threadIdx.x; u < rowM; u += blockDim.x, it.next()) {
             const int y = it.r, m = it.c;
             const int sy = scoord(y, r, g.ny);
-            float v2[2] = {0.f, 0.f};
+            float2 v2 = c2(0.f, 0.f);
             if (sz * sz + sy * sy <= rg2) {
+                if (gi == 0) {
+                    float2 w2 = c2(0.f, 0.f);
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int sx = scoord(2 * m + h, r, g.nx);
-                    if (sz * sz + sy * sy + sx * sx <= rg2) {
-                        float p, w;
-                        gather_pw_rt(tm, g.nz, g.ny, g.nx, R, sz, sy, sx, &p, &w);
-                        v2[h] = gi == 0 ? p : (gi == 1 ? w : w * w);
+                    for (int h = 0; h < 2; ++h) {
+                        const int sx = scoord(2 * m + h, r, g.nx);
+                        if (sz * sz + sy * sy + sx * sx <= rg2) {
+                            float p, w;
+                            gather_pw_rt(tm, g.nz, g.ny, g.nx, R, sz, sy, sx, &p, &w);
+                            if (h) {
+                                v2.y = p;
+                                w2.y = w;
+                            } else {
+                                v2.x = p;
+                                w2.x = w;
+                            }
+                        }
                     }
+                    wst[u] = w2;
+                } else {
+                    const float2 w2 = wst[u];
+                    v2 = gi == 1 ? w2 : c2(w2.x * w2.x, w2.y * w2.y);
                 }
             }
-            A[u] = c2(v2[0], v2[1]);
+            A[u] = v2;
         }
         __syncthreads();
         float2* Z = tile_fft<-1, false>(A, B, px, twx, g.ny, g.M);
@@ -550,6 +567,7 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
         return launched("k_fwd_plane_rt");
     }
     if (which == 1) {
+        if (zcorr_ct_supported(f.nz)) return launch_zcorr_ct(f, a, skip ? zr : f.nz, s);
         const int tiles = (g.plane + kRtTq - 1) / kRtTq;
         const size_t lds = (size_t)(3 * kRtTq + 1) * f.nz * sizeof(float2);
         hipLaunchKernelGGL(k_zcorr_rt, dim3(tiles, (a.nrot + a.chunk_z - 1) / a.chunk_z),

@@ -110,3 +110,25 @@ def test_compile_time_length_passes_match_numpy(host, n):
         assert host.pf_host_ct_fft(n, d, x.ctypes.data_as(FP), out.ctypes.data_as(FP)) == ns
         ref = np.fft.fft(x.astype(np.complex128)) if d < 0 else np.fft.ifft(x) * n
         assert np.abs(out - ref).max() / np.abs(ref).max() < 2e-6, (n, d, radices)
+
+
+@pytest.mark.parametrize("n", [n for n in CT_LENGTHS if n >= 16])
+def test_register_fft_with_stagewise_butterflies_matches_numpy(host, n):
+    """The register FFTs of the run-time planned z stage (each stage with its
+    own butterflies per thread, 16 register slots) equal numpy's transform,
+    forward and inverse, in both stage orders; the forward plan's output
+    pattern is the reversed plan's input pattern and vice versa (the z stage
+    multiplies and transforms back without an exchange).  A wrong idle-slot
+    test or a misplaced output would fail here."""
+    rng = np.random.default_rng(n + 11)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    info = (ctypes.c_int * 4)()
+    for d in (-1, 1):
+        ref = np.fft.fft(x.astype(np.complex128)) if d < 0 else np.fft.ifft(x) * n
+        for rev in (0, 1):
+            out = np.full(n, np.nan, np.complex64)
+            assert host.pf_host_flex_fft(n, d, rev, x.ctypes.data_as(FP),
+                                         out.ctypes.data_as(FP), info) == 0
+            assert np.abs(out - ref).max() / np.abs(ref).max() < 2e-6, (n, d, rev, list(info))
+    t, e, ns, same = list(info)
+    assert same == 1 and e <= 16 and t in (1, 2, 4, 8, 16, 32) and 1 <= ns <= 3
