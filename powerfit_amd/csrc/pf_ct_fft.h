@@ -18,6 +18,8 @@
 // returns false for any other length (the caller then runs the run-time
 // planned pass).
 #pragma once
+#include <type_traits>
+
 #include "pf_group_fft.h"
 
 namespace pf {
@@ -94,17 +96,39 @@ inline float2* ct_host_run(float2* a, float2* b, const float2* tw) {
     }
 }
 
+// First-stage input from a loader instead of the source buffer: element e of
+// transform f is ld(f, e) (e.g. a pre- or post-processing step computed while
+// the stage reads, saving a pass over LDS).
+template <int L, int DIR, int S, class LD>
+PF_HD void ct_bfly_ld(const LD& ld, int f, float2* dp, int j, int es, const float2* tw) {
+    constexpr CtPlan P = ct_plan(L);
+    constexpr int R = P.r[S];
+    constexpr int NB = L / R;
+    static_assert(S == 0, "loaders feed the first stage");
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = ld(f, j + r * NB);
+    dft_any<R, DIR>(v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) dp[(j * R + r) * es] = v[r];
+}
+
+// No loader: every stage reads its source buffer.
+struct CtNoLoad {};
+
 #if defined(__HIP__)
-template <int L, int DIR, bool COLS, int S>
+template <int L, int DIR, bool COLS, int S, class LD>
 __device__ __forceinline__ void ct_stage(const float2* a, float2* b, int nfft, int st,
-                                         const float2* tw) {
+                                         const float2* tw, const LD& ld) {
     constexpr CtPlan P = ct_plan(L);
     constexpr int NB = L / P.r[S];
+    constexpr bool kLd = S == 0 && !std::is_same<LD, CtNoLoad>::value;
     const int total = nfft * NB;
     if constexpr (!COLS) {
         for (int u = threadIdx.x; u < total; u += blockDim.x) {
             const int f = u / NB, j = u - f * NB;
-            ct_bfly<L, DIR, S>(a + f * st, b + f * st, j, 1, tw);
+            if constexpr (kLd) ct_bfly_ld<L, DIR, S>(ld, f, b + f * st, j, 1, tw);
+            else ct_bfly<L, DIR, S>(a + f * st, b + f * st, j, 1, tw);
         }
     } else {
         // u = j nfft + f, stepped without divisions
@@ -112,7 +136,8 @@ __device__ __forceinline__ void ct_stage(const float2* a, float2* b, int nfft, i
         int f = (int)threadIdx.x - j * nfft;
         const int dj = (int)blockDim.x / nfft, df = (int)blockDim.x - dj * nfft;
         for (int u = threadIdx.x; u < total; u += blockDim.x) {
-            ct_bfly<L, DIR, S>(a + f, b + f, j, st, tw);
+            if constexpr (kLd) ct_bfly_ld<L, DIR, S>(ld, f, b + f, j, st, tw);
+            else ct_bfly<L, DIR, S>(a + f, b + f, j, st, tw);
             j += dj;
             f += df;
             if (f >= nfft) {
@@ -123,17 +148,17 @@ __device__ __forceinline__ void ct_stage(const float2* a, float2* b, int nfft, i
     }
 }
 
-template <int L, int DIR, bool COLS, int S = 0>
+template <int L, int DIR, bool COLS, int S = 0, class LD = CtNoLoad>
 __device__ __forceinline__ float2* ct_run(float2* a, float2* b, int nfft, int st,
-                                          const float2* tw) {
+                                          const float2* tw, const LD& ld = LD()) {
     constexpr CtPlan P = ct_plan(L);
     static_assert(P.ns >= 0, "no compile-time plan for this length");
     if constexpr (S >= P.ns) {
         return a;
     } else {
-        ct_stage<L, DIR, COLS, S>(a, b, nfft, st, tw);
+        ct_stage<L, DIR, COLS, S>(a, b, nfft, st, tw, ld);
         __syncthreads();
-        return ct_run<L, DIR, COLS, S + 1>(b, a, nfft, st, tw);
+        return ct_run<L, DIR, COLS, S + 1>(b, a, nfft, st, tw, CtNoLoad());
     }
 }
 #endif
@@ -155,12 +180,15 @@ __device__ __forceinline__ float2* ct_run(float2* a, float2* b, int nfft, int st
 
 #if defined(__HIP__)
 // Run a compiled pass for length L if there is one: *out receives the buffer
-// holding the result.  ROWS lengths for !COLS, column lengths for COLS.
-template <int DIR, bool COLS>
+// holding the result.  ROWS lengths for !COLS, column lengths for COLS.  With
+// a loader the first stage takes its inputs from it (a is then only the
+// second ping-pong buffer).
+template <int DIR, bool COLS, class LD = CtNoLoad>
 __device__ __forceinline__ bool ct_dispatch(int L, float2* a, float2* b, int nfft, int st,
-                                            const float2* tw, float2** out) {
+                                            const float2* tw, float2** out,
+                                            const LD& ld = LD()) {
 #define PF_CT_CASE(n) \
-    case n: *out = ct_run<n, DIR, COLS>(a, b, nfft, st, tw); return true;
+    case n: *out = ct_run<n, DIR, COLS, 0, LD>(a, b, nfft, st, tw, ld); return true;
     if constexpr (COLS) {
         switch (L) {
             PF_CT_COL_LENGTHS(PF_CT_CASE)
@@ -175,7 +203,6 @@ __device__ __forceinline__ bool ct_dispatch(int L, float2* a, float2* b, int nff
 #undef PF_CT_CASE
 }
 #endif
-
 
 // ---------------------------------------------------------------------------
 // Register FFTs of compile-time length L by T threads with stage-by-stage

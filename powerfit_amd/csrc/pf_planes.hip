@@ -32,7 +32,8 @@ namespace pf {
 
 namespace {
 
-constexpr int kRtThreads = 1024;         // plane kernels (one block per CU: the LDS)
+constexpr int kRtThreads = 1024;         // plane kernels: largest block
+constexpr size_t kCuLds = 160 * 1024;    // LDS of one compute unit
 constexpr int kRtVox = 16;               // voxels of a plane owned per thread (K3)
 constexpr int kRtZThreads = 256;         // z kernel
 constexpr int kRtTq = 16;                // z columns per block
@@ -273,10 +274,11 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
     float2* const wst = dst + 2 * g.H;
     const int rowM = g.ny * g.M;
     for (int gi = 0; gi < ngrid; ++gi) {
-        // rows: (v(2m), v(2m+1)) packed as complex, row stride M
-        Step2 it(g.M);
-        for (int u = threadIdx.x; u < rowM; u += blockDim.x, it.next()) {
-            const int y = it.r, m = it.c;
+        // row y, element m: (v(2m), v(2m+1)) packed as complex, row stride M;
+        // gathered for grid 0 (P, with W parked), read back for W and W^2
+        const int M = g.M;
+        auto ldrow = [=](int y, int m) -> float2 {
+            const int u = y * M + m;
             const int sy = scoord(y, r, g.ny);
             float2 v2 = c2(0.f, 0.f);
             if (sz * sz + sy * sy <= rg2) {
@@ -303,27 +305,34 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
                     v2 = gi == 1 ? w2 : c2(w2.x * w2.x, w2.y * w2.y);
                 }
             }
-            A[u] = v2;
+            return v2;
+        };
+        {
+            Step2 it(g.M);
+            for (int u = threadIdx.x; u < rowM; u += blockDim.x, it.next()) A[u] = ldrow(it.r, it.c);
         }
         __syncthreads();
         float2* Z = tile_fft<-1, false>(A, B, px, twx, g.ny, g.M);
         float2* Xp = Z == A ? B : A;
-        // R2C post-processing, rows of nxh: X[k] = E[k] + w^k O[k]
-        Step2 ip(g.nxh);
-        for (int u = threadIdx.x; u < g.plane; u += blockDim.x, ip.next()) {
-            const int y = ip.r, k = ip.c;
-            const float2* z = Z + y * g.M;
-            float2 X;
-            if (k == 0 || k == g.M) {
-                const float2 z0 = z[0];
-                X = c2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.f);
-            } else {
-                const float2 zk = z[k], zm = cconj(z[g.M - k]);
-                const float2 E = cscale(cadd(zk, zm), 0.5f);
-                const float2 O = mul_si<-1>(cscale(csub(zk, zm), 0.5f));
-                X = cadd(E, cmul(twfull[k], O));
+        // R2C post-processing of the rows (row stride M) into rows of nxh:
+        // X[k] = E[k] + w^k O[k], X[0] and X[M] real
+        {
+            Step2 ip(g.nxh);
+            for (int u = threadIdx.x; u < g.plane; u += blockDim.x, ip.next()) {
+                const int k = ip.c;
+                const float2* z = Z + ip.r * M;
+                float2 X;
+                if (k == 0 || k == M) {
+                    const float2 z0 = z[0];
+                    X = c2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.f);
+                } else {
+                    const float2 zk = z[k], zm = cconj(z[M - k]);
+                    const float2 E = cscale(cadd(zk, zm), 0.5f);
+                    const float2 O = mul_si<-1>(cscale(csub(zk, zm), 0.5f));
+                    X = cadd(E, cmul(twfull[k], O));
+                }
+                Xp[u] = X;
             }
-            Xp[u] = X;
         }
         __syncthreads();
         // y FFT of the nxh columns (element stride nxh)
@@ -445,28 +454,41 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
     for (int b = b0; b < b1; ++b) {
         for (int gi = 2; gi >= 0; --gi) {
             const float2* src = X + ((int64_t)b * 3 + gi) * g.H + (int64_t)iz * g.plane;
-            for (int u = threadIdx.x; u < g.plane; u += blockDim.x) A[u] = src[u];
-            __syncthreads();
-            // inverse y over the nxh columns
-            float2* Y = tile_fft<1, true>(A, B, py, twy, g.nxh, g.nxh);
+            // inverse y over the nxh columns; the compiled pass reads the
+            // plane from global memory in its first stage
+            const int nxh0 = g.nxh;
+            auto ldg = [=](int k, int y) -> float2 { return src[y * nxh0 + k]; };
+            float2* Y;
+            if (!ct_dispatch<1, true>(g.ny, B, A, g.nxh, g.nxh, twy, &Y, ldg)) {
+                for (int u = threadIdx.x; u < g.plane; u += blockDim.x) A[u] = src[u];
+                __syncthreads();
+                Y = lds_fft_rt<1>(A, B, py, twy, g.nxh, 1, g.nxh);
+            }
             float2* D = Y == A ? B : A;
-            // C2R pre-processing, rows of M: z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
-            const int rowM = g.ny * g.M;
-            Step2 it(g.M);
-            for (int u = threadIdx.x; u < rowM; u += blockDim.x, it.next()) {
-                const int y = it.r, k = it.c;
-                const float2* xr = Y + y * g.nxh;
-                float2 xk = xr[k], xm = xr[g.M - k];
+            // C2R pre-processing of the rows of nxh into rows of M:
+            // z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]); the
+            // compiled x pass computes it while its first stage reads
+            const int M = g.M, nxh = g.nxh;
+            auto pre = [=](int y, int k) -> float2 {
+                const float2* xr = Y + y * nxh;
+                float2 xk = xr[k], xm = xr[M - k];
                 if (k == 0) {
                     xk.y = 0.f;
                     xm.y = 0.f;
                 }
                 const float2 xmc = cconj(xm);
                 const float2 d = cmul(csub(xk, xmc), cconj(twfull[k]));
-                D[u] = cadd(cadd(xk, xmc), mul_si<1>(d));
+                return cadd(cadd(xk, xmc), mul_si<1>(d));
+            };
+            float2* Z;
+            if (!ct_dispatch<1, false>(g.M, Y, D, g.ny, g.M, twx, &Z, pre)) {
+                const int rowM = g.ny * g.M;
+                Step2 it(g.M);
+                for (int u = threadIdx.x; u < rowM; u += blockDim.x, it.next())
+                    D[u] = pre(it.r, it.c);
+                __syncthreads();
+                Z = lds_fft_rt<1>(D, Y, px, twx, g.ny, g.M, 1);
             }
-            __syncthreads();
-            const float2* Z = tile_fft<1, false>(D, Y, px, twx, g.ny, g.M);
             const float* zr = reinterpret_cast<const float*>(Z);   // row y: 2 M reals
 #pragma unroll
             for (int k = 0; k < kRtVox; ++k) {
@@ -555,13 +577,19 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
     g.plane = f.ny * g.nxh;
     g.H = (int64_t)f.nz * g.plane;
     const size_t lds_plane = plane_lds_bytes(g);
+    // plane kernels: 512-thread blocks where two fit a compute unit's LDS (and
+    // the LCC stage's voxels per thread allow), so one block's barriers
+    // overlap the other's work; otherwise one 1024-thread block per unit
+    const int nthr_plane =
+        (2 * lds_plane <= kCuLds && (int64_t)f.ny * f.nx <= (int64_t)kRtVox * 512) ? 512
+                                                                                   : kRtThreads;
     // planes |z| <= zr only, the z stage reading the others as zero
     const int zr = std::max(0, std::min(a.zr, f.nz / 2));
     const bool skip = 2 * zr + 1 < f.nz;
     if (which == 0) {
         const int ngrid = a.binary_mask ? 2 : 3;
         const int nplanes = skip ? 2 * zr + 1 : f.nz;
-        hipLaunchKernelGGL(k_fwd_plane_rt, dim3(nplanes, a.nrot), dim3(kRtThreads), lds_plane, s,
+        hipLaunchKernelGGL(k_fwd_plane_rt, dim3(nplanes, a.nrot), dim3(nthr_plane), lds_plane, s,
                            a.tm, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, ngrid, a.work, f.px,
                            f.py, f.twx, f.twy, f.twfull);
         return launched("k_fwd_plane_rt");
@@ -576,7 +604,7 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
         return launched("k_zcorr_rt");
     }
     hipLaunchKernelGGL(k_inv_plane_rt, dim3(f.nz, (a.nrot + a.chunk_lcc - 1) / a.chunk_lcc),
-                       dim3(kRtThreads), lds_plane, s, a.work, a.lcc_mask, a.keys, g, a.nrot,
+                       dim3(nthr_plane), lds_plane, s, a.work, a.lcc_mask, a.keys, g, a.nrot,
                        a.chunk_lcc, a.first, a.norm, f.px, f.py, f.twx, f.twy, f.twfull);
     return launched("k_inv_plane_rt");
 }
