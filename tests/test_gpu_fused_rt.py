@@ -39,6 +39,15 @@ CASES = {
                              seed=8103, shape=(70, 72, 98)),
     "s50x60x42_l": Workload("s50x60x42_l", 42, 2.0, 800, 8.0, 20.0, laplace=True, seed=8104,
                             shape=(50, 60, 42)),
+    # z = 44 and x / 2 = 33 carry a factor 11: no compiled transform, so the
+    # z stage and the row passes take their run-time planned forms
+    "s44x48x66": Workload("s44x48x66", 44, 2.0, 800, 8.0, 20.0, seed=8107,
+                          shape=(44, 48, 66)),
+    # odd y length (63 = 9 * 7) with Laplace and core weighting
+    "s60x63x72_lc": Workload("s60x63x72_lc", 60, 2.0, 1200, 8.0, 20.0, laplace=True,
+                             core_weighted=True, seed=8108, shape=(60, 63, 72)),
+    # short z axis: two threads per z-column, blocks below a wavefront
+    "s24x36x40": Workload("s24x36x40", 24, 2.5, 300, 9.0, 20.0, seed=8109, shape=(24, 36, 40)),
 }
 
 
