@@ -3,11 +3,13 @@
 Times short scans of the correlator's own inputs over candidate rotation-batch
 sizes and per-block rotation chunks of the fused kernels, then, at the best
 of those, the other schedules of the fused z stage (alternative FFT radix
-plan, alternative column tile width, both), and keeps the fastest.  Results are cached per (device, grid, mask kind, mode) in a JSON
+plan, alternative column tile width, both), and keeps the fastest.  Results are cached per (device, grid, mask kind, mode, precision) in a JSON
 file (``$POWERFIT_AMD_TUNE`` or ``~/.cache/powerfit_amd/tune.json``).  Batch
 and chunk sizes do not change the result (every rotation's LCC map is computed
 identically and merged with an order-independent max); the FFT schedule
-changes it only by FP32 rounding.
+changes it only by FP32 rounding.  With split precision the fused z stage
+has one schedule (the one whose first stage runs on the matrix cores), so only
+batch and chunks are tuned, and those choices are cached separately.
 """
 import json
 import os
@@ -35,8 +37,9 @@ def _device_name():
 
 def _key(corr):
     binary = bool(np.all((corr.mask == 0) | (corr.mask == 1)))
+    precision = getattr(corr, "precision", "fp32")
     return f"{_device_name()}|{'x'.join(map(str, corr.target.shape))}|" \
-           f"{'binary' if binary else 'weighted'}|{corr.mode}"
+           f"{'binary' if binary else 'weighted'}|{corr.mode}|{precision}"
 
 
 def _time(corr, batch, chunks, nrot, repeats=2, z_schedule=0):
@@ -94,7 +97,9 @@ def autotune(corr, batches=DEFAULT_BATCHES, chunks=DEFAULT_CHUNKS, nrot=None, us
                       flush=True)
     best = max(trials, key=lambda t: t["rate"])
     best["z_schedule"] = 0
-    if schedules and corr.fused_available and corr.mode in ("auto", "fused"):
+    # split precision runs the fused z stage's default schedule only
+    if (schedules and corr.fused_available and corr.mode in ("auto", "fused")
+            and getattr(corr, "precision", "fp32") != "split"):
         for zs in schedules:
             if zs == 0:
                 continue

@@ -117,10 +117,13 @@ class HipCorrelator(BaseCorrelator):
     batch : rotations per launch (256 measured best at every fused size; a
         scan of fewer rotations uses only as many).
     binary_shortcut : use FFT(W^2) = FFT(W) when the mask is 0/1.
-    precision : "fp32" (default) or "split": the unfused y and z FFT passes
-        do their first radix-16 stage as a DFT-as-GEMM on the matrix cores with
-        FP16 hi + lo operands and FP32 accumulation (ny, nz divisible by 16 and
-        at most 320; the fused pipeline is not used).
+    precision : "fp32" (default) or "split": the first radix-16 stage of an
+        FFT runs as a DFT-as-GEMM on the matrix cores with FP16 hi + lo
+        operands and FP32 accumulation -- in the unfused y and z passes (ny, nz
+        divisible by 16 and at most 320), and in the fused pipeline's forward z
+        transforms at the compiled sizes whose z plan starts with radix 16
+        (128^3); other fused sizes and the run-time planned pipeline are not
+        used with it.
     """
 
     def __init__(self, target, laplace=False, device=0, mode="auto", batch=256,

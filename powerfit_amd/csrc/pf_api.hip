@@ -217,10 +217,11 @@ int plan_scan(pf_ctx* c, int64_t first, int64_t count, int batch, int mode, Scan
     if (first < 0 || count < 0 || first + count > c->nrot)
         return arg_error("rotation range outside the rotation set");
     if (mode < PF_MODE_PLAIN || mode > PF_MODE_FUSED_RT) return arg_error("unknown mode");
-    // the split-precision FFT stage exists in the unfused passes only
+    // the split-precision FFT stage: the unfused y and z passes, or the fused
+    // z stage at the compiled sizes that have its form
     const bool split = (c->flags & PF_FLAG_SPLIT_DFT) != 0;
-    const bool fused_ok =
-        !split && fused_supported(c->nz, c->ny, c->nx) && c->rmax <= c->nx / 2;
+    const bool fused_ok = fused_supported(c->nz, c->ny, c->nx) && c->rmax <= c->nx / 2 &&
+                          (!split || fused_split_supported(c->nx));
     const bool rt_ok = !split && rt_available(c);
     if (mode == PF_MODE_AUTO)
         mode = fused_ok ? PF_MODE_FUSED : (rt_ok ? PF_MODE_FUSED_RT : PF_MODE_BATCHED);
@@ -271,7 +272,9 @@ int enqueue_scan(pf_ctx* c, int64_t first, int64_t count, const ScanPlan& sp, fl
     fa_args.occ = c->d_occ;
     fa_args.rmax = c->rmax;
     fa_args.zr = c->support_zr;
-    fa_args.zsched = c->zsched;
+    // split precision: the z stage's default schedule, with its MFMA first stage
+    fa_args.split_z = (c->flags & PF_FLAG_SPLIT_DFT) != 0;
+    fa_args.zsched = fa_args.split_z ? 0 : c->zsched;
     fa_args.skip_planes = fa && fb && !sp.rt;
     fa_args.work = cplx;
     fa_args.F1 = c->d_F1;
@@ -719,7 +722,8 @@ int64_t pf_ctx_bytes(int nz, int ny, int nx, int batch, int mode, uint32_t flags
                           fused_supported(nz, ny, nx);
     const bool rt = (mode == PF_MODE_FUSED_RT || mode == PF_MODE_AUTO) &&
                     plane_rt_supported(nz, ny, nx);
-    const bool fused = !(flags & PF_FLAG_SPLIT_DFT) && (compiled || rt);
+    const bool split = (flags & PF_FLAG_SPLIT_DFT) != 0;
+    const bool fused = (compiled && (!split || fused_split_supported(nx))) || (rt && !split);
     // template, mask (f32), packed pair, keys, lcc mask, two target spectra,
     // brick occupancy, rotations
     int64_t bytes = V * (4 + 4 + 8 + 8 + 1) + 2 * H * 8 + nrot * 9 * 8;
@@ -837,8 +841,11 @@ int pf_scan_graph(pf_ctx* c, int64_t first, int64_t count, int batch, int mode) 
 
 int pf_ctx_fused_available(pf_ctx* c) {
     if (!c) return arg_error("ctx is null");
-    if (c->flags & PF_FLAG_SPLIT_DFT) return 0;
-    if (fused_supported(c->nz, c->ny, c->nx) && (c->rmax < 0 || c->rmax <= c->nx / 2)) return 1;
+    const bool split = (c->flags & PF_FLAG_SPLIT_DFT) != 0;
+    if (fused_supported(c->nz, c->ny, c->nx) && (c->rmax < 0 || c->rmax <= c->nx / 2) &&
+        (!split || fused_split_supported(c->nx)))
+        return 1;
+    if (split) return 0;
     return rt_available(c) ? 2 : 0;
 }
 

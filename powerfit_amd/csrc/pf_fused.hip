@@ -30,6 +30,7 @@
 #include <type_traits>
 
 #include "pf_group_fft.h"
+#include "pf_mfma.h"
 #include "pf_internal.h"
 #include "pf_rotate.h"
 
@@ -332,13 +333,23 @@ __global__ __launch_bounds__(NTHR) void k_fwd_plane(const float2* __restrict__ t
 }
 
 // ---------------------------------------------------------------------------
-template <int N, int TQ, int EMAX, bool BIN>
+// SPLIT: the forward z FFTs do their first stage, one radix-16 butterfly per
+// thread, as a split-precision DFT-GEMM on the matrix cores (pf_mfma.h): a
+// wavefront holds butterfly rows j0 = t & ~1 and j0 + 1 of its 32 columns,
+// and each of the two 32 x 32 GEMMs takes one row, lane (c, h = t & 1)
+// holding inputs 4 h + q and 8 + 4 h + q of that row's butterfly in column
+// c.  The remaining stages, the products and the inverse transforms stay
+// FP32 on the VALU.
+template <int N, int TQ, int EMAX, bool BIN, bool SPLIT = false>
 __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
     float2* __restrict__ X, const float2* __restrict__ F1, const float2* __restrict__ F2,
     int nrot, int chunk, int zr, const float2* __restrict__ twN) {
     using G = Geo<N>;
     constexpr int TZ = pick_T(N, EMAX, 32);
     constexpr int EZ = N / TZ;
+    static_assert(!SPLIT || (TQ == 32 && TZ % 2 == 0 && EZ == 16 &&
+                             gplan_for(N, TZ).r[0] == 16),
+                  "split first stage: one radix-16 butterfly per thread, 32-column tiles");
     // lanes run across columns (coalesced column loads); stages use
     // workgroup barriers (wave-local columns cost more in scattered loads)
     __shared__ float2 ex[N * TQ];
@@ -359,22 +370,51 @@ __global__ __launch_bounds__(TQ * pick_T(N, EMAX, 32)) void k_zcorr(
         }
     }
     float2* exc = ex + c;
+    half4 fh[4], fl[4];
+    if constexpr (SPLIT) dft16_operand<-1>(fh, fl);
     for (int b = b0; b < b1; ++b) {
         float2* base = X + (int64_t)b * 3 * G::H + q;
 #pragma unroll
         for (int g = 0; g < (BIN ? 2 : 3); ++g) {
             float2* col = base + (int64_t)g * G::H;
+            // planes beyond the support were not written: they are zero
+            auto ld = [&](int z) -> float2 {
+                const int sz = z <= N / 2 ? z : z - N;
+                return (sz >= -zr && sz <= zr) ? col[(int64_t)z * G::PLANE] : c2(0.f, 0.f);
+            };
             float2 v[EZ];
-            if (active) {
+            if constexpr (SPLIT) {
+                constexpr int NB = N / 16;   // stage-0 butterflies per column
+                const int h = t & 1, j0 = t & ~1;
 #pragma unroll
-                for (int e = 0; e < EZ; ++e) {
-                    // planes beyond the support were not written: they are zero
-                    const int z = t + TZ * e;
-                    const int sz = z <= N / 2 ? z : z - N;
-                    v[e] = (sz >= -zr && sz <= zr) ? col[(int64_t)z * G::PLANE] : c2(0.f, 0.f);
+                for (int jj = 0; jj < 2; ++jj) {
+                    const int j = j0 + jj;
+                    float2 x[8];
+#pragma unroll
+                    for (int qq = 0; qq < 4; ++qq) {
+                        x[qq] = active ? ld(j + NB * (4 * h + qq)) : c2(0.f, 0.f);
+                        x[4 + qq] = active ? ld(j + NB * (8 + 4 * h + qq)) : c2(0.f, 0.f);
+                    }
+                    dft16_split_gemm(x, fh, fl);
+                    if (active) {
+#pragma unroll
+                        for (int qq = 0; qq < 4; ++qq) {
+                            exc[(16 * j + 4 * h + qq) * TQ] = x[qq];
+                            exc[(16 * j + 8 + 4 * h + qq) * TQ] = x[4 + qq];
+                        }
+                    }
                 }
+                __syncthreads();
+                if (active) GroupStage<N, TZ, -1, 0>::read_phase(v, exc, TQ, t);
+                __syncthreads();
+                GroupStage<N, TZ, -1, 1>::run(v, exc, TQ, t, stw, active);
+            } else {
+                if (active) {
+#pragma unroll
+                    for (int e = 0; e < EZ; ++e) v[e] = ld(t + TZ * e);
+                }
+                group_fft<N, TZ, -1>(v, exc, TQ, t, stw, active);
             }
-            group_fft<N, TZ, -1>(v, exc, TQ, t, stw, active);
             if (BIN && g == 1) {
                 // W spectrum feeds both A (with F1) and B (with F2)
                 float2 u[EZ];
@@ -1093,17 +1133,25 @@ void launch_fwd_pair(dim3 grid, hipStream_t s, const float2* tm, const double* r
 }
 
 // z stage with element bound EZ (its radix schedule: pick_T(N, EZ, 32)).
-template <int N, int TQ, int EZ>
+template <int N, int TQ, int EZ, bool SPLIT = false>
 void launch_zcorr(const FusedArgs& a, int ch, int zr_read, const float2* twN, hipStream_t s) {
     using G = Geo<N>;
     const dim3 grid((G::PLANE + TQ - 1) / TQ, (a.nrot + ch - 1) / ch);
     const int nthr = TQ * pick_T(N, EZ, 32);
     if (a.binary_mask)
-        hipLaunchKernelGGL((k_zcorr<N, TQ, EZ, true>), grid, dim3(nthr), 0, s, a.work, a.F1, a.F2,
-                           a.nrot, ch, zr_read, twN);
-    else
-        hipLaunchKernelGGL((k_zcorr<N, TQ, EZ, false>), grid, dim3(nthr), 0, s, a.work, a.F1,
+        hipLaunchKernelGGL((k_zcorr<N, TQ, EZ, true, SPLIT>), grid, dim3(nthr), 0, s, a.work, a.F1,
                            a.F2, a.nrot, ch, zr_read, twN);
+    else
+        hipLaunchKernelGGL((k_zcorr<N, TQ, EZ, false, SPLIT>), grid, dim3(nthr), 0, s, a.work,
+                           a.F1, a.F2, a.nrot, ch, zr_read, twN);
+}
+
+// The split z stage exists where the default schedule's z plan starts with
+// radix 16 over one butterfly per thread in 32-column tiles.
+template <int N, int TQ, int EZ>
+constexpr bool zsplit_ok() {
+    return TQ == 32 && pick_T(N, EZ, 32) % 2 == 0 && N / pick_T(N, EZ, 32) == 16 &&
+           gplan_for(N, pick_T(N, EZ, 32)).r[0] == 16;
 }
 
 // Launch shape of one fused size.  K1: NTHR1 threads, element bounds EMAX (rows,
@@ -1182,7 +1230,15 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     if (which < 0 || which == 1) {
         const int ch = a.chunk_z;
         const bool alt_plan = a.zsched & 1, alt_tile = a.zsched & 2;
-        if (alt_tile && alt_plan)
+        if (a.split_z) {
+            // split-precision first stage: the default schedule only
+            if constexpr (zsplit_ok<N, TQ, SH::EMAXZ>()) {
+                launch_zcorr<N, TQ, SH::EMAXZ, true>(a, ch, zr_read, tw.N, s);
+            } else {
+                set_error("fused z stage: no split-precision form for n = %d", N);
+                return kErrUnsupported;
+            }
+        } else if (alt_tile && alt_plan)
             launch_zcorr<N, SH::TQ2, SH::EMAXZ2>(a, ch, zr_read, tw.N, s);
         else if (alt_tile)
             launch_zcorr<N, SH::TQ2, SH::EMAXZ>(a, ch, zr_read, tw.N, s);
@@ -1217,6 +1273,21 @@ bool fused_supported(int nz, int ny, int nx) {
     switch (nx) {
         case 48: case 64: case 72: case 80: case 96: case 128: case 144: case 160: case 192:
             return true;
+        default: return false;
+    }
+}
+
+bool fused_split_supported(int n) {
+    switch (n) {
+        case 48: return zsplit_ok<48, 32, 12>();
+        case 64: return zsplit_ok<64, 32, 8>();
+        case 72: return zsplit_ok<72, Shape72::TQ, Shape72::EMAXZ>();
+        case 80: return zsplit_ok<80, Shape80::TQ, Shape80::EMAXZ>();
+        case 96: return zsplit_ok<96, 32, 12>();
+        case 128: return zsplit_ok<128, Shape128::TQ, Shape128::EMAXZ>();
+        case 144: return zsplit_ok<144, Shape144::TQ, Shape144::EMAXZ>();
+        case 160: return zsplit_ok<160, Shape160::TQ, Shape160::EMAXZ>();
+        case 192: return zsplit_ok<192, Shape192::TQ, Shape192::EMAXZ>();
         default: return false;
     }
 }

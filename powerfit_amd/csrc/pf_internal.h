@@ -125,7 +125,11 @@ struct FusedArgs {
     unsigned long long* keys;
     double norm, invV;
     int chunk_z, chunk_lcc;      // rotations per block in the z and LCC kernels
+    bool split_z;                // forward z FFTs' first stage as a split-precision
+                                 // DFT-GEMM on MFMA (fused_split_supported sizes)
 };
+// Compiled sizes whose fused z stage has the split-precision MFMA first stage.
+bool fused_split_supported(int n);
 // Enqueue the three fused stages; `which` selects the stage (0: rotate +
 // forward xy, 1: z correlate, 2: inverse xy + LCC) or -1 for all three.
 int launch_fused(const FusedArgs& a, int which, hipStream_t s);

@@ -15,30 +15,13 @@
 #include <math.h>
 
 #include "pf_internal.h"
+#include "pf_mfma.h"
 
 namespace pf {
 namespace {
 
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-
 constexpr int kTq = 16;        // tile columns; a GEMM block is 2 stage rows j of 16 columns
 constexpr int kThreads = 256;  // 4 waves
-
-// Element (i, k) of the 32 x 32 real form of the 16-point DFT with sign DIR.
-template <int DIR>
-__device__ __forceinline__ float dft16_real(int i, int k) {
-    const int m = ((i & 15) * (k & 15)) & 15;
-    const double c = cospi((double)m / 8.0);
-    const double d = (double)DIR * sinpi((double)m / 8.0);
-    if (i < 16) return (float)(k < 16 ? c : -d);
-    return (float)(k < 16 ? d : c);
-}
-
-__device__ __forceinline__ void split_half(float v, _Float16& hi, _Float16& lo) {
-    hi = (_Float16)v;
-    lo = (_Float16)(v - (float)hi);
-}
 
 // Stages s0.. of plan p over LDS data (as lds_fft, starting at span Ns0).
 template <int DIR>
@@ -94,17 +77,8 @@ __global__ __launch_bounds__(kThreads) void k_c2c_axis_split(float2* __restrict_
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane & 31, h = lane >> 5;
     const int c = g & 15;
-    // A operand (F16, rows i = lane % 32), K step s holds columns 8 s + 4 h + q
     half4 fh[4], fl[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            _Float16 hi, lo;
-            split_half(dft16_real<DIR>(g, 8 * s + 4 * h + q), hi, lo);
-            fh[s][q] = hi;
-            fl[s][q] = lo;
-        }
+    dft16_operand<DIR>(fh, fl);
     for (int jb = wave; jb < (L + 1) / 2; jb += kThreads / 64) {
         const int j = 2 * jb + (g >> 4);
         const bool valid = j < L;
@@ -116,46 +90,12 @@ __global__ __launch_bounds__(kThreads) void k_c2c_axis_split(float2* __restrict_
             x[q] = A[(jr + (4 * h + q) * L) * kTq + c];
             x[4 + q] = A[(jr + (8 + 4 * h + q) * L) * kTq + c];
         }
-        float mx = 0.f;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) mx = fmaxf(mx, fmaxf(fabsf(x[q].x), fabsf(x[q].y)));
-        mx = fmaxf(mx, __shfl_xor(mx, 32, 64));   // the column's other half
-        int e = 0;
-        if (mx > 0.f && isfinite(mx)) (void)frexpf(mx, &e);
-        const float sc = ldexpf(1.f, -e), unsc = ldexpf(1.f, e);
-        // B operand, K step s: rows 8 s + 4 h + q = Re x[4h+q], Re x[8+4h+q],
-        // Im x[4h+q], Im x[8+4h+q] for s = 0..3
-        half4 xh[4], xl[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            _Float16 hi, lo;
-            split_half(x[q].x * sc, hi, lo);
-            xh[0][q] = hi;
-            xl[0][q] = lo;
-            split_half(x[4 + q].x * sc, hi, lo);
-            xh[1][q] = hi;
-            xl[1][q] = lo;
-            split_half(x[q].y * sc, hi, lo);
-            xh[2][q] = hi;
-            xl[2][q] = lo;
-            split_half(x[4 + q].y * sc, hi, lo);
-            xh[3][q] = hi;
-            xl[3][q] = lo;
-        }
-        floatx16 acc = {};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(fh[s], xh[s], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(fh[s], xl[s], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(fl[s], xh[s], acc, 0, 0, 0);
-        }
-        // result rows 8 (v / 4) + 4 h + v % 4: Re y[4h+q] = acc[q],
-        // Re y[8+4h+q] = acc[4+q], Im y[4h+q] = acc[8+q], Im y[8+4h+q] = acc[12+q]
+        dft16_split_gemm(x, fh, fl);
         if (valid) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                B[(16 * j + 4 * h + q) * kTq + c] = c2(acc[q] * unsc, acc[8 + q] * unsc);
-                B[(16 * j + 8 + 4 * h + q) * kTq + c] = c2(acc[4 + q] * unsc, acc[12 + q] * unsc);
+                B[(16 * j + 4 * h + q) * kTq + c] = x[q];
+                B[(16 * j + 8 + 4 * h + q) * kTq + c] = x[4 + q];
             }
         }
     }

@@ -44,8 +44,14 @@ def test_ctx_bytes_model():
     unfused = lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["batched"], 0, K)
     plain = lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["plain"], 0, K)
     split = lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["auto"], _lib.FLAG_SPLIT_DFT, K)
-    assert fused == resident + 3 * b * H * 8
-    assert unfused == split == resident + 3 * b * (H * 8 + V * 4)
+    assert fused == split == resident + 3 * b * H * 8   # 128^3: split z stage fused
+    assert unfused == resident + 3 * b * (H * 8 + V * 4)
+    # a fused size without the split z stage runs split precision unfused
+    n3 = 96
+    V3, H3 = n3 ** 3, n3 * n3 * (n3 // 2 + 1)
+    res3 = V3 * 25 + 2 * H3 * 8 + K * 72 + 4 * (((n3 // 8) ** 3 + 31) // 32)
+    assert lib.pf_ctx_bytes(n3, n3, n3, b, _lib.MODES["auto"], _lib.FLAG_SPLIT_DFT, K) == \
+        res3 + 3 * b * (H3 * 8 + V3 * 4)
     assert plain == resident + max(3 * (H * 8 + V * 4), 16 * V)
     # a range shorter than the batch uses only as many rotations per launch
     assert lib.pf_ctx_bytes(n, n, n, b, _lib.MODES["fused"], 0, 3) == \
