@@ -144,9 +144,9 @@ def cpu_baseline_check(case, Rs, lcc, rot, nproc=1, ref=None):
 
 def precision_report(args, HipCorrelator, case, R, w):
     """FP32 vs split-precision (MFMA first stage) unfused passes on a rotation
-    prefix, and the fused FP32 pipeline where the grid has one: rotations/s
-    with inputs resident, and every result checked against the FP64 oracle in
-    the CPU leg."""
+    prefix, and the fused pipeline where the grid has one (FP32, and split
+    where its z stage has the MFMA form): rotations/s with inputs resident,
+    and every result checked against the FP64 oracle in the CPU leg."""
     n = min(args.precision_rotations, R.shape[0])
     Rs = R[:n]
     rows = {}
@@ -156,6 +156,11 @@ def precision_report(args, HipCorrelator, case, R, w):
                           mode="batched", batch=1)
     if probe.fused_available:
         runs.append(("fp32_fused", "fp32", "fused"))
+    del probe
+    probe = HipCorrelator(case.target.astype(np.float64), laplace=case.laplace, device=0,
+                          mode="batched", batch=1, precision="split")
+    if probe.fused_available:
+        runs.append(("split_fused", "split", "fused"))
     del probe
     for key, prec, mode in runs:
         corr = HipCorrelator(case.target.astype(np.float64), laplace=case.laplace, device=0,
