@@ -49,11 +49,14 @@ extern "C" {
 
 /* pf_ctx_create flags */
 #define PF_FLAG_NO_BINARY_MASK 1u /* never use the W^2 = W shortcut for 0/1 masks  */
-/* Split-precision FFT: the y and z passes of the unfused pipeline do their
- * first radix-16 stage as a DFT-as-GEMM on the matrix cores (FP16 hi + lo
- * operands, FP32 accumulation); needs ny, nz divisible by 16 and <= 320.
- * The fused pipeline is not used with this flag (AUTO runs the batched
- * kernels).  No counterpart in the reference: its precision option. */
+/* Split-precision FFT: the first radix-16 stage runs as a DFT-as-GEMM on the
+ * matrix cores (FP16 hi + lo operands, FP32 accumulation) -- in the y and z
+ * passes of the unfused pipeline (needs ny, nz divisible by 16 and <= 320),
+ * and in the forward z transforms of the fused pipeline at the compiled sizes
+ * whose z plan starts with radix 16 (128^3).  With this flag AUTO runs the
+ * fused pipeline there and the batched kernels elsewhere; the run-time planned
+ * pipeline is not used.  No counterpart in the reference: its precision
+ * option. */
 #define PF_FLAG_SPLIT_DFT 2u
 
 int pf_abi_version(void);
