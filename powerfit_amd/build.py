@@ -18,8 +18,8 @@ HOST_LIB = os.path.join(HERE, "libpf_fft_host.so")
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["pf_kernels.hip", "pf_fused.hip", "pf_planes.hip", "pf_zct.hip", "pf_api.hip",
-               "pf_shapes.hip", "pf_peaks.hip", "pf_split.hip"]
+HIP_SOURCES = ["pf_kernels.hip", "pf_fused.hip", "pf_planes.hip", "pf_planes_odd.hip",
+               "pf_zct.hip", "pf_api.hip", "pf_shapes.hip", "pf_peaks.hip", "pf_split.hip"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 NO_PACKED_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -110,8 +110,9 @@ class HipCorrelator(BaseCorrelator):
             "batched" (``batch`` rotations per launch, unfused kernels),
             "fused" (``batch`` rotations per launch through the three fused
             kernels; cubic grids of a compiled size), "fused_rt" (the three
-            fused stages with run-time FFT plans: other grids with even nx
-            whose planes fit LDS), or "auto" (fused, else fused_rt where
+            fused stages with the grid shape taken at run time: other grids
+            whose planes fit LDS, at most 16,384 voxels each), or "auto"
+            (fused, else fused_rt where
             available, else batched).  "mix_fwd"/"mix_z"/"mix_inv" run one
             fused stage with the rest unfused (for testing).
     batch : rotations per launch (256 measured best at every fused size; a

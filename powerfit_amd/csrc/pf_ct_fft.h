@@ -178,12 +178,17 @@ __device__ __forceinline__ float2* ct_run(float2* a, float2* b, int nfft, int st
     X(147) X(150) X(160) X(162) X(168) X(175) X(180) X(189) X(192) X(196) X(200) X(210) X(216) \
     X(224) X(225) X(240) X(243) X(245) X(250) X(252) X(256)
 
+// The odd row lengths (whole odd x lengths, two packed rows per transform).
+#define PF_CT_ODD_LENGTHS(X) \
+    X(9) X(15) X(21) X(25) X(27) X(35) X(45) X(49) X(63) X(75) X(81) X(105) X(125) X(135) \
+    X(147) X(175) X(189) X(225) X(243) X(245)
+
 #if defined(__HIP__)
 // Run a compiled pass for length L if there is one: *out receives the buffer
-// holding the result.  ROWS lengths for !COLS, column lengths for COLS.  With
-// a loader the first stage takes its inputs from it (a is then only the
-// second ping-pong buffer).
-template <int DIR, bool COLS, class LD = CtNoLoad>
+// holding the result.  Column lengths for COLS, else row lengths (the odd
+// ones only with ODDROWS).  With a loader the first stage takes its inputs
+// from it (a is then only the second ping-pong buffer).
+template <int DIR, bool COLS, bool ODDROWS = false, class LD = CtNoLoad>
 __device__ __forceinline__ bool ct_dispatch(int L, float2* a, float2* b, int nfft, int st,
                                             const float2* tw, float2** out,
                                             const LD& ld = LD()) {
@@ -192,6 +197,11 @@ __device__ __forceinline__ bool ct_dispatch(int L, float2* a, float2* b, int nff
     if constexpr (COLS) {
         switch (L) {
             PF_CT_COL_LENGTHS(PF_CT_CASE)
+            default: return false;
+        }
+    } else if constexpr (ODDROWS) {
+        switch (L) {
+            PF_CT_ODD_LENGTHS(PF_CT_CASE)
             default: return false;
         }
     } else {

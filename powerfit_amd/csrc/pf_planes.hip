@@ -1,6 +1,6 @@
 // Runtime-planned fused LCC pipeline for grids without a compiled fused size
-// (non-cubic grids, and cubic lengths whose radices the register FFTs cannot
-// split evenly over threads, e.g. 120 = 8 * 3 * 5).
+// (non-cubic grids, odd x lengths, and cubic lengths whose radices the
+// register FFTs cannot split evenly over threads, e.g. 120 = 8 * 3 * 5).
 //
 // The same three stages as the compiled pipeline (pf_fused.hip), with the FFT
 // lengths and radix plans taken at run time: every transform is an LDS
@@ -20,328 +20,14 @@
 //                   outputs, merged into the running (LCC, index) keys with
 //                   atomicMax.
 //
-// Limits (plane_rt_supported): even nx; two planes of ny x (nx/2+1) complex
-// in LDS; at most kRtVox voxels of a plane per thread of the LCC stage.
-#include <algorithm>
-
-#include "pf_ct_fft.h"
-#include "pf_internal.h"
-#include "pf_rotate.h"
+// Limits (plane_rt_supported): two planes of ny x (nx/2+1) complex (for odd
+// nx, or its packed rows if larger) in LDS; at most kRtVox voxels of a plane
+// per thread of the LCC stage.
+#include "pf_planes_kern.h"
 
 namespace pf {
 
 namespace {
-
-constexpr int kRtThreads = 1024;         // plane kernels: largest block
-constexpr size_t kCuLds = 160 * 1024;    // LDS of one compute unit
-constexpr int kRtVox = 16;               // voxels of a plane owned per thread (K3)
-constexpr int kRtZThreads = 256;         // z kernel
-constexpr int kRtTq = 16;                // z columns per block
-constexpr size_t kRtLds = 150 * 1024;    // dynamic LDS a plane kernel may take
-
-// Signed coordinate of index i on an axis of length n for rotation radius
-// r <= n / 2: i for i <= r, else i - n (as the compiled pipeline's).
-__device__ __forceinline__ int scoord(int i, int r, int n) { return i <= r ? i : i - n; }
-
-// Rotated template (trilinear, FP32 blend) and mask (nearest) at signed voxel
-// (z, y, x) from the interleaved (template, mask) grid, any shape.  FP64
-// coordinates, floor and round in the reference's order (so corner and
-// nearest indices are the reference's), one "+n" wrap for negatives, the
-// upper neighbour wrapping only at n.
-__device__ __forceinline__ void gather_pw_rt(const float2* __restrict__ tm, int nz, int ny,
-                                             int nx, const double* R, int z, int y, int x,
-                                             float* p, float* w) {
-    const Coord c = source_coord(R, z, y, x);
-    const double fx = floor(c.xs), fy = floor(c.ys), fz = floor(c.zs);
-    const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
-    const int xi = x0 + (x0 < 0 ? nx : 0), yi = y0 + (y0 < 0 ? ny : 0), zi = z0 + (z0 < 0 ? nz : 0);
-    *p = 0.f;
-    *w = 0.f;
-    if ((unsigned)xi >= (unsigned)nx || (unsigned)yi >= (unsigned)ny || (unsigned)zi >= (unsigned)nz)
-        return;
-    const int xj = (xi + 1 == nx) ? 0 : xi + 1;
-    const int yj = (yi + 1 == ny) ? 0 : yi + 1;
-    const int zj = (zi + 1 == nz) ? 0 : zi + 1;
-    const float2* r00 = tm + ((int64_t)zi * ny + yi) * nx;
-    const float2* r10 = tm + ((int64_t)zi * ny + yj) * nx;
-    const float2* r01 = tm + ((int64_t)zj * ny + yi) * nx;
-    const float2* r11 = tm + ((int64_t)zj * ny + yj) * nx;
-    const float2 a000 = r00[xi], a001 = r00[xj], a010 = r10[xi], a011 = r10[xj];
-    const float2 a100 = r01[xi], a101 = r01[xj], a110 = r11[xi], a111 = r11[xj];
-    const float dx = (float)(c.xs - fx), dy = (float)(c.ys - fy), dz = (float)(c.zs - fz);
-    const float c00 = a000.x + (a001.x - a000.x) * dx;
-    const float c10 = a010.x + (a011.x - a010.x) * dx;
-    const float c01 = a100.x + (a101.x - a100.x) * dx;
-    const float c11 = a110.x + (a111.x - a110.x) * dx;
-    const float c0 = c00 + (c10 - c00) * dy;
-    const float c1 = c01 + (c11 - c01) * dy;
-    *p = c0 + (c1 - c0) * dz;
-    // round() is half away from zero: up from floor when the fraction
-    // exceeds one half, or equals it for non-negative values
-    const double hx = c.xs - fx, hy = c.ys - fy, hz = c.zs - fz;
-    const bool ux = hx > 0.5 || (hx == 0.5 && c.xs >= 0.0);
-    const bool uy = hy > 0.5 || (hy == 0.5 && c.ys >= 0.0);
-    const bool uz = hz > 0.5 || (hz == 0.5 && c.zs >= 0.0);
-    const float2 n0 = uy ? (ux ? a011 : a010) : (ux ? a001 : a000);
-    const float2 n1 = uy ? (ux ? a111 : a110) : (ux ? a101 : a100);
-    *w = uz ? n1.y : n0.y;
-}
-
-// One Stockham butterfly of radix R (the stage of pf_fft.h) with its twiddle
-// phase k = j mod Ns and twiddle step n / (Ns R) passed in, so a butterfly
-// needs no integer division.
-template <int R, int DIR>
-__device__ __forceinline__ void bfly_k(const float2* sp, float2* dp, int j, int k, int Ns, int nb,
-                                       int step, int es, const float2* tw) {
-    float2 v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = sp[(j + r * nb) * es];
-#pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<DIR>(tw, r * k * step));
-    dft_fixed<R, DIR>(v);
-    const int od = (j - k) * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) dp[(od + r * Ns) * es] = v[r];
-}
-
-template <int DIR>
-__device__ __noinline__ void bfly_k_any(const float2* sp, float2* dp, int R, int j, int k, int Ns,
-                                        int nb, int step, int es, const float2* tw) {
-    const int od = (j - k) * R + k;
-    for (int q = 0; q < R; ++q) {
-        float2 acc = c2(0.f, 0.f);
-        for (int m = 0; m < R; ++m) {
-            float2 x = sp[(j + m * nb) * es];
-            if (m > 0) x = cmul(x, twid<DIR>(tw, m * k * step));
-            const int e = (m * q) % R;
-            acc = cadd(acc, e ? cmul(x, twid<DIR>(tw, e * nb)) : x);
-        }
-        dp[(od + q * Ns) * es] = acc;
-    }
-}
-
-// One stage of lds_fft (pf_fft.h) for radix R (0: any radix, generic DFT),
-// with each thread stepping through its butterflies by the block size:
-// (transform f, butterfly j) and k = j mod Ns advance by adds and compares
-// (nb is a multiple of Ns, so wrapping j past nb keeps k).
-template <int R, int DIR>
-__device__ __forceinline__ void rt_stage(const float2* a, float2* b, int Rr, int Ns, int n,
-                                         const float2* tw, int nfft, int fs, int es) {
-    const int T = blockDim.x, t = threadIdx.x;
-    const int RR = R > 0 ? R : Rr;
-    const int nb = n / RR, step = n / (Ns * RR), total = nfft * nb;
-    int f, j, df, dj;
-    if (es == 1) {   // u = f nb + j
-        f = t / nb;
-        j = t - f * nb;
-        df = T / nb;
-        dj = T - df * nb;
-    } else {         // u = j nfft + f
-        j = t / nfft;
-        f = t - j * nfft;
-        dj = T / nfft;
-        df = T - dj * nfft;
-    }
-    int k = j % Ns;
-    const int dk = dj % Ns;
-    for (int u = t; u < total; u += T) {
-        const float2* sp = a + f * fs;
-        float2* dp = b + f * fs;
-        if constexpr (R > 0) bfly_k<R, DIR>(sp, dp, j, k, Ns, nb, step, es, tw);
-        else bfly_k_any<DIR>(sp, dp, RR, j, k, Ns, nb, step, es, tw);
-        k += dk;
-        if (k >= Ns) k -= Ns;
-        if (es == 1) {
-            j += dj;
-            f += df;
-            if (j >= nb) {
-                j -= nb;
-                ++f;
-            }
-        } else {
-            f += df;
-            j += dj;
-            if (f >= nfft) {
-                f -= nfft;
-                ++j;
-                if (++k >= Ns) k -= Ns;
-            }
-        }
-    }
-}
-
-// lds_fft (pf_fft.h) over nfft transforms in LDS with the radix of each stage
-// dispatched once per stage; twiddles are read from LDS.  Returns the buffer
-// holding the result; ends with a barrier.
-template <int DIR>
-__device__ float2* lds_fft_rt(float2* a, float2* b, const Plan1D& p, const float2* tw, int nfft,
-                              int fs, int es) {
-    const int n = p.n;
-    int Ns = 1;
-    for (int s = 0; s < p.nstages; ++s) {
-        const int R = p.radix[s];
-        switch (R) {
-            case 2: rt_stage<2, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
-            case 3: rt_stage<3, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
-            case 4: rt_stage<4, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
-            case 5: rt_stage<5, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
-            case 7: rt_stage<7, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
-            case 8: rt_stage<8, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
-            default: rt_stage<0, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
-        }
-        __syncthreads();
-        float2* tmp = a;
-        a = b;
-        b = tmp;
-        Ns *= R;
-    }
-    return a;
-}
-
-// Transforms of a plane or column tile: the compiled pass for the length
-// when there is one (pf_ct_fft.h), else the run-time planned one.  COLS:
-// transform f at a[f + e st]; otherwise at a[f st + e].
-template <int DIR, bool COLS>
-__device__ __forceinline__ float2* tile_fft(float2* a, float2* b, const Plan1D& p,
-                                            const float2* tw, int nfft, int st) {
-    float2* out;
-    if (ct_dispatch<DIR, COLS>(p.n, a, b, nfft, st, tw, &out)) return out;
-    return lds_fft_rt<DIR>(a, b, p, tw, nfft, COLS ? 1 : st, COLS ? st : 1);
-}
-
-// Row-major stepping over an (rows x cols) index space by the block size:
-// (r, c) of u = threadIdx.x, advanced by adds and compares.
-struct Step2 {
-    int r, c, dr, dc, cols;
-    __device__ __forceinline__ Step2(int cols_) : cols(cols_) {
-        r = (int)threadIdx.x / cols;
-        c = (int)threadIdx.x - r * cols;
-        dr = (int)blockDim.x / cols;
-        dc = (int)blockDim.x - dr * cols;
-    }
-    __device__ __forceinline__ void next() {
-        r += dr;
-        c += dc;
-        if (c >= cols) {
-            c -= cols;
-            ++r;
-        }
-    }
-};
-
-// Copy a twiddle table into LDS.
-__device__ __forceinline__ void stage_tw(float2* dst, const float2* __restrict__ src, int n) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
-}
-
-struct RtGeo {
-    int nz, ny, nx, M, nxh, plane;   // M = nx / 2, nxh = M + 1, plane = ny * nxh
-    int64_t H;
-};
-
-// LDS of a plane kernel: two planes of ny x nxh complex, then the twiddles
-// of the x (M), y (ny) and R2C/C2R (nx) tables.
-__host__ __device__ inline size_t plane_lds_bytes(const RtGeo& g) {
-    return sizeof(float2) * ((size_t)2 * g.plane + g.M + g.ny + g.nx);
-}
-
-// Forward stage.  Planes zlo, zlo + 1, ... (mod nz) of the support.
-__global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
-    const float2* __restrict__ tm, const double* __restrict__ rots, RtGeo g, int r, int zr,
-    int zlo, int ngrid, float2* __restrict__ out, Plan1D px, Plan1D py,
-    const float2* __restrict__ twx_g, const float2* __restrict__ twy_g,
-    const float2* __restrict__ twf_g) {
-    extern __shared__ float2 lds[];
-    float2* const A = lds;
-    float2* const B = lds + g.plane;
-    float2* const twx = lds + 2 * g.plane;
-    float2* const twy = twx + g.M;
-    float2* const twfull = twy + g.ny;
-    stage_tw(twx, twx_g, g.M);
-    stage_tw(twy, twy_g, g.ny);
-    stage_tw(twfull, twf_g, g.nx);
-    const int iz = (blockIdx.x + zlo + g.nz) % g.nz, b = blockIdx.y;
-    const int sz = scoord(iz, r, g.nz);
-    // gathered voxels: inside the rotation sphere and the support ball
-    const int rg = r < zr ? r : zr, rg2 = rg * rg;
-    double R[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
-    float2* dst = out + (int64_t)b * 3 * g.H + (int64_t)iz * g.plane;
-    // The mask values gathered with P are parked in this plane's region of
-    // grid 2 (written last, by this block only, after every read of it; each
-    // thread reads back only what it wrote) for the W (and W^2) grids.
-    float2* const wst = dst + 2 * g.H;
-    const int rowM = g.ny * g.M;
-    for (int gi = 0; gi < ngrid; ++gi) {
-        // row y, element m: (v(2m), v(2m+1)) packed as complex, row stride M;
-        // gathered for grid 0 (P, with W parked), read back for W and W^2
-        const int M = g.M;
-        auto ldrow = [=](int y, int m) -> float2 {
-            const int u = y * M + m;
-            const int sy = scoord(y, r, g.ny);
-            float2 v2 = c2(0.f, 0.f);
-            if (sz * sz + sy * sy <= rg2) {
-                if (gi == 0) {
-                    float2 w2 = c2(0.f, 0.f);
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int sx = scoord(2 * m + h, r, g.nx);
-                        if (sz * sz + sy * sy + sx * sx <= rg2) {
-                            float p, w;
-                            gather_pw_rt(tm, g.nz, g.ny, g.nx, R, sz, sy, sx, &p, &w);
-                            if (h) {
-                                v2.y = p;
-                                w2.y = w;
-                            } else {
-                                v2.x = p;
-                                w2.x = w;
-                            }
-                        }
-                    }
-                    wst[u] = w2;
-                } else {
-                    const float2 w2 = wst[u];
-                    v2 = gi == 1 ? w2 : c2(w2.x * w2.x, w2.y * w2.y);
-                }
-            }
-            return v2;
-        };
-        {
-            Step2 it(g.M);
-            for (int u = threadIdx.x; u < rowM; u += blockDim.x, it.next()) A[u] = ldrow(it.r, it.c);
-        }
-        __syncthreads();
-        float2* Z = tile_fft<-1, false>(A, B, px, twx, g.ny, g.M);
-        float2* Xp = Z == A ? B : A;
-        // R2C post-processing of the rows (row stride M) into rows of nxh:
-        // X[k] = E[k] + w^k O[k], X[0] and X[M] real
-        {
-            Step2 ip(g.nxh);
-            for (int u = threadIdx.x; u < g.plane; u += blockDim.x, ip.next()) {
-                const int k = ip.c;
-                const float2* z = Z + ip.r * M;
-                float2 X;
-                if (k == 0 || k == M) {
-                    const float2 z0 = z[0];
-                    X = c2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.f);
-                } else {
-                    const float2 zk = z[k], zm = cconj(z[M - k]);
-                    const float2 E = cscale(cadd(zk, zm), 0.5f);
-                    const float2 O = mul_si<-1>(cscale(csub(zk, zm), 0.5f));
-                    X = cadd(E, cmul(twfull[k], O));
-                }
-                Xp[u] = X;
-            }
-        }
-        __syncthreads();
-        // y FFT of the nxh columns (element stride nxh)
-        const float2* Y = tile_fft<-1, true>(Xp, Z, py, twy, g.nxh, g.nxh);
-        float2* o = dst + (int64_t)gi * g.H;
-        for (int u = threadIdx.x; u < g.plane; u += blockDim.x) o[u] = Y[u];
-        __syncthreads();
-    }
-}
 
 // z stage.  LDS: S0, S1, S2 (nz x TQ complex each, element (z, c) at z TQ + c)
 // and the z twiddles; the F1, F2 tiles are read from global memory (L2) by
@@ -421,128 +107,11 @@ __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
     }
 }
 
-// Inverse stage and LCC.  Each thread owns voxels u = threadIdx.x + k * blockDim.x
-// (k < kRtVox) of the plane, row-major (y, x); it keeps B, then 1 / sqrt(N V B
-// - A^2), of them in registers.
-__global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
-    const float2* __restrict__ X, const uint8_t* __restrict__ lmask,
-    unsigned long long* __restrict__ keys, RtGeo g, int nrot, int chunk, int64_t first,
-    double norm, Plan1D px, Plan1D py, const float2* __restrict__ twx_g,
-    const float2* __restrict__ twy_g, const float2* __restrict__ twf_g) {
-    extern __shared__ float2 lds[];
-    float2* const A = lds;
-    float2* const B = lds + g.plane;
-    float2* const twx = lds + 2 * g.plane;
-    float2* const twy = twx + g.M;
-    float2* const twfull = twy + g.ny;
-    stage_tw(twx, twx_g, g.M);
-    stage_tw(twy, twy_g, g.ny);
-    stage_tw(twfull, twf_g, g.nx);
-    const int iz = blockIdx.x;
-    const int b0 = blockIdx.y * chunk, b1 = min(nrot, b0 + chunk);
-    const int nvox = g.ny * g.nx;
-    const double nv = norm * ((double)g.nz * g.ny * g.nx);   // N V (exact)
-    const int64_t vz = (int64_t)iz * nvox;
-    static_assert(kRtVox <= 32, "mask bits of the owned voxels fit a 32-bit word");
-    uint32_t mbits = 0;
-#pragma unroll
-    for (int k = 0; k < kRtVox; ++k) {
-        const int u = threadIdx.x + k * blockDim.x;
-        if (u < nvox && lmask[vz + u]) mbits |= 1u << k;
-    }
-    float held[kRtVox];
-    for (int b = b0; b < b1; ++b) {
-        for (int gi = 2; gi >= 0; --gi) {
-            const float2* src = X + ((int64_t)b * 3 + gi) * g.H + (int64_t)iz * g.plane;
-            // inverse y over the nxh columns; the compiled pass reads the
-            // plane from global memory in its first stage
-            const int nxh0 = g.nxh;
-            auto ldg = [=](int k, int y) -> float2 { return src[y * nxh0 + k]; };
-            float2* Y;
-            if (!ct_dispatch<1, true>(g.ny, B, A, g.nxh, g.nxh, twy, &Y, ldg)) {
-                for (int u = threadIdx.x; u < g.plane; u += blockDim.x) A[u] = src[u];
-                __syncthreads();
-                Y = lds_fft_rt<1>(A, B, py, twy, g.nxh, 1, g.nxh);
-            }
-            float2* D = Y == A ? B : A;
-            // C2R pre-processing of the rows of nxh into rows of M:
-            // z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]); the
-            // compiled x pass computes it while its first stage reads
-            const int M = g.M, nxh = g.nxh;
-            auto pre = [=](int y, int k) -> float2 {
-                const float2* xr = Y + y * nxh;
-                float2 xk = xr[k], xm = xr[M - k];
-                if (k == 0) {
-                    xk.y = 0.f;
-                    xm.y = 0.f;
-                }
-                const float2 xmc = cconj(xm);
-                const float2 d = cmul(csub(xk, xmc), cconj(twfull[k]));
-                return cadd(cadd(xk, xmc), mul_si<1>(d));
-            };
-            float2* Z;
-            if (!ct_dispatch<1, false>(g.M, Y, D, g.ny, g.M, twx, &Z, pre)) {
-                const int rowM = g.ny * g.M;
-                Step2 it(g.M);
-                for (int u = threadIdx.x; u < rowM; u += blockDim.x, it.next())
-                    D[u] = pre(it.r, it.c);
-                __syncthreads();
-                Z = lds_fft_rt<1>(D, Y, px, twx, g.ny, g.M, 1);
-            }
-            const float* zr = reinterpret_cast<const float*>(Z);   // row y: 2 M reals
-#pragma unroll
-            for (int k = 0; k < kRtVox; ++k) {
-                const int u = threadIdx.x + k * blockDim.x;
-                if (u >= nvox) continue;
-                const float val = zr[u];   // (y, x) -> real 2 (y M) + x = u
-                if (gi == 2) {
-                    held[k] = val;
-                } else if (gi == 1) {
-                    const double ad = (double)val;
-                    held[k] = __builtin_amdgcn_rsqf((float)fma(-ad, ad, (double)held[k] * nv));
-                } else if ((mbits >> k) & 1u) {
-                    // only S > 0 can beat the initial (0, 0); NaN never wins
-                    const float sv = val * held[k];
-                    if (sv > 0.f) atomicMax(keys + vz + u, lcc_key(sv, (uint32_t)(first + b)));
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-int launched(const char* what) {
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        set_error("launch of %s failed: %s", what, hipGetErrorString(e));
-        return kErrHip;
-    }
-    return kOk;
-}
-
-// Let a kernel take more than the default dynamic LDS (once per process and
-// device; the attribute is per function).
-int allow_lds(const void* fn, size_t bytes) {
-    const hipError_t e =
-        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize, %zu): %s", bytes,
-                  hipGetErrorString(e));
-        return kErrHip;
-    }
-    return kOk;
-}
-
 }  // namespace
 
 bool plane_rt_supported(int nz, int ny, int nx) {
-    if (nx % 2 != 0 || nz < 2 || ny < 2 || nx < 4) return false;
-    RtGeo g;
-    g.ny = ny;
-    g.nx = nx;
-    g.M = nx / 2;
-    g.plane = ny * (g.M + 1);
+    if (nz < 2 || ny < 2 || nx < 3) return false;
+    const RtGeo g = make_geo(nz, ny, nx);
     if (plane_lds_bytes(g) > kRtLds) return false;
     if ((int64_t)ny * nx > (int64_t)kRtVox * kRtThreads) return false;
     // z kernel: three column tiles and the z twiddles
@@ -551,9 +120,9 @@ bool plane_rt_supported(int nz, int ny, int nx) {
 }
 
 int prepare_planes_rt() {
-    PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt, kRtLds));
+    PF_CHECK_STATUS(prepare_plane_kernels<false>());
+    PF_CHECK_STATUS(prepare_plane_kernels_odd());
     PF_CHECK_STATUS(allow_lds((const void*)k_zcorr_rt, kRtLds));
-    PF_CHECK_STATUS(allow_lds((const void*)k_inv_plane_rt, kRtLds));
     return kOk;
 }
 
@@ -568,34 +137,12 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
         set_error("fused: invalid batch or chunk sizes");
         return kErrArg;
     }
-    RtGeo g;
-    g.nz = f.nz;
-    g.ny = f.ny;
-    g.nx = f.nx;
-    g.M = f.nx / 2;
-    g.nxh = g.M + 1;
-    g.plane = f.ny * g.nxh;
-    g.H = (int64_t)f.nz * g.plane;
-    const size_t lds_plane = plane_lds_bytes(g);
-    // plane kernels: 512-thread blocks where two fit a compute unit's LDS (and
-    // the LCC stage's voxels per thread allow), so one block's barriers
-    // overlap the other's work; otherwise one 1024-thread block per unit
-    const int nthr_plane =
-        (2 * lds_plane <= kCuLds && (int64_t)f.ny * f.nx <= (int64_t)kRtVox * 512) ? 512
-                                                                                   : kRtThreads;
     // planes |z| <= zr only, the z stage reading the others as zero
     const int zr = std::max(0, std::min(a.zr, f.nz / 2));
     const bool skip = 2 * zr + 1 < f.nz;
-    if (which == 0) {
-        const int ngrid = a.binary_mask ? 2 : 3;
-        const int nplanes = skip ? 2 * zr + 1 : f.nz;
-        hipLaunchKernelGGL(k_fwd_plane_rt, dim3(nplanes, a.nrot), dim3(nthr_plane), lds_plane, s,
-                           a.tm, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, ngrid, a.work, f.px,
-                           f.py, f.twx, f.twy, f.twfull);
-        return launched("k_fwd_plane_rt");
-    }
     if (which == 1) {
         if (zcorr_ct_supported(f.nz)) return launch_zcorr_ct(f, a, skip ? zr : f.nz, s);
+        const RtGeo g = make_geo(f.nz, f.ny, f.nx);
         const int tiles = (g.plane + kRtTq - 1) / kRtTq;
         const size_t lds = (size_t)(3 * kRtTq + 1) * f.nz * sizeof(float2);
         hipLaunchKernelGGL(k_zcorr_rt, dim3(tiles, (a.nrot + a.chunk_z - 1) / a.chunk_z),
@@ -603,10 +150,8 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
                            a.binary_mask ? 1 : 0, skip ? zr : f.nz, f.pz, f.twz);
         return launched("k_zcorr_rt");
     }
-    hipLaunchKernelGGL(k_inv_plane_rt, dim3(f.nz, (a.nrot + a.chunk_lcc - 1) / a.chunk_lcc),
-                       dim3(nthr_plane), lds_plane, s, a.work, a.lcc_mask, a.keys, g, a.nrot,
-                       a.chunk_lcc, a.first, a.norm, f.px, f.py, f.twx, f.twy, f.twfull);
-    return launched("k_inv_plane_rt");
+    if (f.nx % 2) return launch_plane_kernel_odd(f, a, which, skip, zr, s);
+    return launch_plane_kernel<false>(f, a, which, skip, zr, s);
 }
 
 }  // namespace pf

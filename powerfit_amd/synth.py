@@ -61,6 +61,9 @@ WORKLOADS = {
                               shape=(84, 96, 112)),
     "grid70x72x98": Workload("grid70x72x98", 70, 1.6, 4000, 6.5, 10.0, seed=6200,
                              shape=(70, 72, 98)),
+    # odd lengths on every axis (x = 105: two rows per x transform)
+    "grid75x81x105": Workload("grid75x81x105", 75, 1.5, 5000, 6.0, 10.0, seed=6300,
+                              shape=(75, 81, 105)),
 }
 
 

@@ -66,7 +66,12 @@ def test_ctx_bytes_model():
         V2 * 25 + 2 * H2 * 8 + 3 * b * H2 * 8
     assert lib.pf_ctx_bytes(n2, n2, n2, b, _lib.MODES["fused_rt"], 0, 0) == \
         V2 * 25 + 2 * H2 * 8 + 3 * b * H2 * 8
-    for shape in ((125, 125, 125), (200, 200, 200)):   # odd x; planes beyond LDS
+    # odd x (two rows per x transform) within the planes' limits: spectra only
+    V4, H4 = 125 ** 3, 125 * 125 * 63
+    assert lib.pf_ctx_bytes(125, 125, 125, b, _lib.MODES["auto"], 0, 0) == \
+        V4 * 25 + 2 * H4 * 8 + 3 * b * H4 * 8
+    # planes of more than 16,384 voxels (odd and even x): the unfused layout
+    for shape in ((135, 135, 135), (200, 200, 200)):
         nz, ny, nx = shape
         V3, H3 = nz * ny * nx, nz * ny * (nx // 2 + 1)
         assert lib.pf_ctx_bytes(nz, ny, nx, b, _lib.MODES["auto"], 0, 0) == \

@@ -1,8 +1,8 @@
 """GPU parity of the run-time planned fused pipeline (PF_MODE_FUSED_RT): grids
 without a compiled fused size -- 120^3 (8 * 3 * 5) and the non-cubic,
-radix-7 shapes the powerfit command line produces (``extend`` to the next
-2,3,5,7-smooth length per axis, powerfit.py:231-233) -- against the FP64
-oracle, with 0/1, core-weighted and Laplace-filtered inputs.
+radix-7 and odd-length shapes the powerfit command line produces (``extend``
+to the next 2,3,5,7-smooth length per axis, powerfit.py:231-233) -- against
+the FP64 oracle, with 0/1, core-weighted and Laplace-filtered inputs.
 
 Bar: best LCC within 1e-4 of the oracle; rotation indices identical where
 the oracle's winner leads by more than 1e-4.
@@ -48,6 +48,15 @@ CASES = {
                              core_weighted=True, seed=8108, shape=(60, 63, 72)),
     # short z axis: two threads per z-column, blocks below a wavefront
     "s24x36x40": Workload("s24x36x40", 24, 2.5, 300, 9.0, 20.0, seed=8109, shape=(24, 36, 40)),
+    # odd x lengths: rows y = 2p, 2p + 1 packed into one complex transform;
+    # odd y as well (the last row unpaired), Laplace and core weighting
+    "s60x64x75_cw": Workload("s60x64x75_cw", 60, 2.0, 1200, 8.0, 20.0, core_weighted=True,
+                             seed=8110, shape=(60, 64, 75)),
+    "s49x45x81_l": Workload("s49x45x81_l", 45, 2.0, 900, 8.0, 20.0, laplace=True, seed=8111,
+                            shape=(49, 45, 81)),
+    "s48x50x45": Workload("s48x50x45", 45, 2.0, 600, 8.0, 20.0, seed=8106, shape=(48, 50, 45)),
+    # odd x length with a factor 11 (the run-time planned row pass)
+    "s40x42x55": Workload("s40x42x55", 40, 2.0, 500, 8.0, 20.0, seed=8112, shape=(40, 42, 55)),
 }
 
 
@@ -116,10 +125,10 @@ def test_fused_rt_on_a_compiled_size_matches_oracle():
     assert np.abs(rt.lcc - fu.lcc).max() <= 1e-4
 
 
-def test_odd_x_length_falls_back_to_batched():
-    """Odd nx has no half-length x transform: AUTO runs the unfused kernels,
-    fused_rt is refused."""
-    c = make_case(Workload("odd", 45, 2.0, 600, 8.0, 20.0, seed=8106, shape=(48, 50, 45)))
+def test_large_plane_falls_back_to_batched():
+    """A plane of more than 16,384 voxels (130 x 130) is beyond the LCC stage's
+    voxels per thread: AUTO runs the unfused kernels, fused_rt is refused."""
+    c = make_case(Workload("big", 24, 2.0, 600, 8.0, 20.0, seed=8113, shape=(24, 130, 130)))
     R = rotations_for_angle(20.0)[:6]
     corr = scan(c, R, "auto")
     assert not corr.fused_rt_available and not corr.fused_available
@@ -127,6 +136,23 @@ def test_odd_x_length_falls_back_to_batched():
     assert_scan_parity(corr.lcc, corr.rot, ref)
     with pytest.raises(RuntimeError):
         scan(c, R, "fused_rt")
+
+
+def test_odd_x_batches_and_ranges_agree():
+    """Odd x: batch sizes, chunks and rotation ranges do not change a result,
+    and AUTO takes the run-time planned pipeline."""
+    c = make_case(CASES["s60x64x75_cw"])
+    R = rotations_for_angle(20.0)[:10]
+    a = scan(c, R, "auto", batch=10)
+    assert a.fused_rt_available
+    b = scan(c, R, "fused_rt", batch=3)
+    assert np.array_equal(a.lcc, b.lcc) and np.array_equal(a.rot, b.rot)
+    b.set_tuning(1, 1)
+    b.reset()
+    b.enqueue_scan(0, 4)
+    b.enqueue_scan(4, 6)
+    lcc, rot = b.result()
+    assert np.array_equal(lcc, a.lcc) and np.array_equal(rot, a.rot)
 
 
 def test_fused_rt_graph_and_noalloc_equal_eager():
