@@ -41,9 +41,10 @@ extern "C" {
 #define PF_MODE_MIX_FWD 4  /* fused rotate + forward xy                             */
 #define PF_MODE_MIX_Z 5    /* fused z correlate                                     */
 #define PF_MODE_MIX_INV 6  /* fused inverse xy + LCC                                */
-/* The three fused stages with run-time FFT plans (LDS Stockham passes): grids
- * without a compiled fused size (pf_ctx_fused_available returns 2): even nx,
- * an ny x (nx/2+1) complex plane twice in LDS, ny nx <= 16384.  PF_MODE_AUTO
+/* The three fused stages with the grid shape taken at run time (FFT passes
+ * compiled per length): grids without a compiled fused size
+ * (pf_ctx_fused_available returns 2) whose ny x (nx/2+1) complex plane (for odd
+ * nx, its rows packed in pairs) fits LDS twice, ny nx <= 16384.  PF_MODE_AUTO
  * takes it where the compiled pipeline does not exist. */
 #define PF_MODE_FUSED_RT 7
 
