@@ -11,10 +11,23 @@
 #include <utility>
 #include <vector>
 
+#include "pf_ct_fft.h"
 #include "pf_internal.h"
 #include "pf_rotate.h"
 
 namespace pf {
+
+// nfft transforms of p.n points in LDS, ping-ponging between a and b: the
+// pass compiled for the length (pf_ct_fft.h) where there is one, else the
+// run-time planned lds_fft.  COLS: transform f at a[f + e st]; otherwise at
+// a[f st + e].  Returns the buffer holding the result; ends with a barrier.
+template <int DIR, bool COLS>
+__device__ __forceinline__ const float2* axis_fft(float2* a, float2* b, const Plan1D& p,
+                                                  const float2* tw, int nfft, int st) {
+    float2* out;
+    if (ct_dispatch<DIR, COLS>(p.n, a, b, nfft, st, tw, &out)) return out;
+    return lds_fft<DIR>(a, b, p, tw, nfft, COLS ? 1 : st, COLS ? st : 1);
+}
 
 // ---------------------------------------------------------------------------
 // R2C along the contiguous x axis: rows of nx reals -> nx/2+1 complex.
@@ -36,7 +49,7 @@ __global__ void k_r2c_x(const float* __restrict__ in, float2* __restrict__ out, 
         A[f * M + m] = even ? c2(src[2 * m], src[2 * m + 1]) : c2(src[m], 0.f);
     }
     __syncthreads();
-    const float2* Z = lds_fft<-1>(A, B, p, tw, nrows, M, 1);
+    const float2* Z = axis_fft<-1, false>(A, B, p, tw, nrows, M);
     for (int u = threadIdx.x; u < nrows * nxh; u += blockDim.x) {
         const int f = u / nxh, k = u - f * nxh;
         const float2* z = Z + f * M;
@@ -96,7 +109,7 @@ __global__ void k_c2r_x(const float2* __restrict__ in, float* __restrict__ out, 
         A[f * M + k] = z;
     }
     __syncthreads();
-    const float2* Z = lds_fft<1>(A, B, p, tw, nrows, M, 1);
+    const float2* Z = axis_fft<1, false>(A, B, p, tw, nrows, M);
     for (int u = threadIdx.x; u < nrows * M; u += blockDim.x) {
         const int f = u / M, m = u - f * M;
         float* dst = out + (row0 + f) * nx;
@@ -132,7 +145,7 @@ __global__ void k_c2c_axis(float2* __restrict__ data, int64_t stride, int64_t in
         A[k * tq + c] = base[k * stride + c];
     }
     __syncthreads();
-    const float2* Z = lds_fft<DIR>(A, B, p, tw, nc, 1, tq);
+    const float2* Z = axis_fft<DIR, true>(A, B, p, tw, nc, tq);
     for (int u = threadIdx.x; u < n * nc; u += blockDim.x) {
         const int k = u / nc, c = u - k * nc;
         base[k * stride + c] = Z[k * tq + c];
