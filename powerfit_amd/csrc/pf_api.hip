@@ -34,7 +34,7 @@ struct pf_ctx {
     float2* d_F2 = nullptr;       // F[Tf^2]
     uint8_t* d_lccmask = nullptr;
     float2* d_tm = nullptr;       // packed (template, mask) for the fused path
-    uint32_t* d_occ = nullptr;    // brick occupancy of d_tm (fused grids only)
+    uint32_t* d_occ = nullptr;    // brick occupancy of d_tm (8^3 bricks)
     unsigned long long* d_keys = nullptr;  // running (LCC, index) keys
     uint32_t flags = 0;
     bool mask_binary = false;
@@ -502,10 +502,7 @@ int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** o
     if (!s) s = dev_alloc(&c->d_F2, c->H);
     if (!s) s = dev_alloc(&c->d_lccmask, c->V);
     if (!s) s = dev_alloc(&c->d_tm, c->V);
-    if (!s && fused_supported(nz, ny, nx)) {
-        const int64_t nb = nx / 8;
-        s = dev_alloc(&c->d_occ, (size_t)((nb * nb * nb + 31) / 32));
-    }
+    if (!s) s = dev_alloc(&c->d_occ, (size_t)occupancy_words(nz, ny, nx));
     if (!s) s = dev_alloc(&c->d_keys, c->V);
     if (s) {
         pf_ctx_destroy(c);
@@ -610,7 +607,7 @@ int pf_set_template_mask(pf_ctx* c, const double* tmpl, const double* mask, doub
     }
     if (!s) s = launch_convert(dt, c->d_mask, nullptr, c->V, c->stream);
     if (!s) s = launch_pack_tm(c->d_template, c->d_mask, c->d_tm, c->V, c->stream);
-    if (!s && c->d_occ) s = launch_brick_occupancy(c->d_tm, c->nx, c->d_occ, c->stream);
+    if (!s) s = launch_brick_occupancy(c->d_tm, c->nz, c->ny, c->nx, c->d_occ, c->stream);
     if (!s && hipStreamSynchronize(c->stream) != hipSuccess) {
         set_error("set_template_mask: stream sync failed");
         s = kErrHip;
@@ -727,7 +724,7 @@ int64_t pf_ctx_bytes(int nz, int ny, int nx, int batch, int mode, uint32_t flags
     // template, mask (f32), packed pair, keys, lcc mask, two target spectra,
     // brick occupancy, rotations
     int64_t bytes = V * (4 + 4 + 8 + 8 + 1) + 2 * H * 8 + nrot * 9 * 8;
-    if (fused_supported(nz, ny, nx)) bytes += 4 * (((int64_t)(nx / 8) * (nx / 8) * (nx / 8) + 31) / 32);
+    bytes += 4 * (int64_t)occupancy_words(nz, ny, nx);
     // work buffers (allocated by the first scan) or, before that, the
     // transient buffers of pf_set_target (f64 upload + two f32 grids) and
     // pf_set_template_mask (f64 upload), whichever is larger

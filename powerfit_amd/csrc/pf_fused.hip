@@ -1050,20 +1050,22 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane2(const float2* __restrict__ 
     }
 }
 
-// Brick occupancy for gather_pw: bit b set when any voxel of 8^3 brick b, or
-// of the one-voxel layer after it in +x/+y/+z (periodic), has a nonzero
-// template or mask value.
-__global__ void k_brick_occupancy(const float2* __restrict__ tm, int n, uint32_t* __restrict__ occ) {
-    const int nb = n / 8;
+// Brick occupancy for the gathers: bit b set when any voxel of 8^3 brick b
+// (bricks ceil(n / 8) per axis, the last one partial), or of the one-voxel
+// layer after it in +x/+y/+z (periodic), has a nonzero template or mask value.
+// A corner cube whose lower corner lies in a clear brick is all zero.
+__global__ void k_brick_occupancy(const float2* __restrict__ tm, int nz, int ny, int nx,
+                                  uint32_t* __restrict__ occ) {
+    const int nbz = (nz + 7) / 8, nby = (ny + 7) / 8, nbx = (nx + 7) / 8;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb * nb * nb) return;
-    const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+    if (b >= nbz * nby * nbx) return;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
     bool any = false;
     for (int dz = 0; dz <= 8 && !any; ++dz)
         for (int dy = 0; dy <= 8 && !any; ++dy)
             for (int dx = 0; dx <= 8; ++dx) {
-                const int z = (bz * 8 + dz) % n, y = (by * 8 + dy) % n, x = (bx * 8 + dx) % n;
-                const float2 v = tm[((int64_t)z * n + y) * n + x];
+                const int z = (bz * 8 + dz) % nz, y = (by * 8 + dy) % ny, x = (bx * 8 + dx) % nx;
+                const float2 v = tm[((int64_t)z * ny + y) * nx + x];
                 if (v.x != 0.f || v.y != 0.f) {
                     any = true;
                     break;
@@ -1318,14 +1320,17 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
     }
 }
 
-int launch_brick_occupancy(const float2* tm, int n, uint32_t* occ, hipStream_t s) {
-    if (n % 8 != 0) {
-        set_error("brick occupancy needs n divisible by 8");
-        return kErrArg;
-    }
-    const int nb = n / 8, nbricks = nb * nb * nb;
-    PF_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(uint32_t) * ((nbricks + 31) / 32), s));
-    hipLaunchKernelGGL(k_brick_occupancy, dim3((nbricks + 255) / 256), dim3(256), 0, s, tm, n, occ);
+int occupancy_words(int nz, int ny, int nx) {
+    const int64_t nb = (int64_t)((nz + 7) / 8) * ((ny + 7) / 8) * ((nx + 7) / 8);
+    return (int)((nb + 31) / 32);
+}
+
+int launch_brick_occupancy(const float2* tm, int nz, int ny, int nx, uint32_t* occ,
+                           hipStream_t s) {
+    const int nbricks = ((nz + 7) / 8) * ((ny + 7) / 8) * ((nx + 7) / 8);
+    PF_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(uint32_t) * occupancy_words(nz, ny, nx), s));
+    hipLaunchKernelGGL(k_brick_occupancy, dim3((nbricks + 255) / 256), dim3(256), 0, s, tm, nz,
+                       ny, nx, occ);
     return last("k_brick_occupancy");
 }
 

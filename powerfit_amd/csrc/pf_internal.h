@@ -148,8 +148,11 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
 bool zcorr_ct_supported(int nz);
 int launch_zcorr_ct(const Fft3D& f, const FusedArgs& a, int zr, hipStream_t s);
 
-// 1 bit per 8^3 brick of an n^3 interleaved grid (n % 8 == 0): ((n/8)^3 + 31) / 32 words.
-int launch_brick_occupancy(const float2* tm, int n, uint32_t* occ, hipStream_t s);
+// 1 bit per 8^3 brick of an interleaved grid (ceil(n / 8) bricks per axis):
+// occupancy_words(nz, ny, nx) words.
+int occupancy_words(int nz, int ny, int nx);
+int launch_brick_occupancy(const float2* tm, int nz, int ny, int nx, uint32_t* occ,
+                           hipStream_t s);
 
 // f64 -> f32 conversion, optionally also writing the square (computed in f64).
 int launch_convert(const double* in, float* out, float* out_sq, int64_t n, hipStream_t s);

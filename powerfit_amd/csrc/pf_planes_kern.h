@@ -29,8 +29,11 @@ __device__ __forceinline__ int scoord(int i, int r, int n) { return i <= r ? i :
 // (z, y, x) from the interleaved (template, mask) grid, any shape.  FP64
 // coordinates, floor and round in the reference's order (so corner and
 // nearest indices are the reference's), one "+n" wrap for negatives, the
-// upper neighbour wrapping only at n.
-__device__ __forceinline__ void gather_pw_rt(const float2* __restrict__ tm, int nz, int ny,
+// upper neighbour wrapping only at n.  Sources whose corner cube starts in a
+// clear 8^3 brick of occ (launch_brick_occupancy) are exactly zero and skip
+// the loads.
+__device__ __forceinline__ void gather_pw_rt(const float2* __restrict__ tm,
+                                             const uint32_t* __restrict__ occ, int nz, int ny,
                                              int nx, const double* R, int z, int y, int x,
                                              float* p, float* w) {
     const Coord c = source_coord(R, z, y, x);
@@ -41,6 +44,8 @@ __device__ __forceinline__ void gather_pw_rt(const float2* __restrict__ tm, int 
     *w = 0.f;
     if ((unsigned)xi >= (unsigned)nx || (unsigned)yi >= (unsigned)ny || (unsigned)zi >= (unsigned)nz)
         return;
+    const int brick = ((zi >> 3) * ((ny + 7) >> 3) + (yi >> 3)) * ((nx + 7) >> 3) + (xi >> 3);
+    if (!((occ[brick >> 5] >> (brick & 31)) & 1u)) return;
     const int xj = (xi + 1 == nx) ? 0 : xi + 1;
     const int yj = (yi + 1 == ny) ? 0 : yi + 1;
     const int zj = (zi + 1 == nz) ? 0 : zi + 1;
@@ -254,7 +259,8 @@ __host__ __device__ inline size_t plane_lds_bytes(const RtGeo& g) {
 // Forward stage.  Planes zlo, zlo + 1, ... (mod nz) of the support.
 template <bool ODD>
 __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
-    const float2* __restrict__ tm, const double* __restrict__ rots, RtGeo g, int r, int zr,
+    const float2* __restrict__ tm, const uint32_t* __restrict__ occ,
+    const double* __restrict__ rots, RtGeo g, int r, int zr,
     int zlo, int ngrid, float2* __restrict__ out, Plan1D px, Plan1D py,
     const float2* __restrict__ twx_g, const float2* __restrict__ twy_g,
     const float2* __restrict__ twf_g) {
@@ -302,7 +308,7 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
                             const int sx = scoord(2 * m + h, r, g.nx);
                             if (sz * sz + sy * sy + sx * sx <= rg2) {
                                 float p, w;
-                                gather_pw_rt(tm, g.nz, g.ny, g.nx, R, sz, sy, sx, &p, &w);
+                                gather_pw_rt(tm, occ, g.nz, g.ny, g.nx, R, sz, sy, sx, &p, &w);
                                 if (h) {
                                     v2.y = p;
                                     w2.y = w;
@@ -327,7 +333,7 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
                 if (sz * sz + sy * sy + sx * sx > rg2) return 0.f;
                 if (gi == 0) {
                     float p, w;
-                    gather_pw_rt(tm, g.nz, g.ny, g.nx, R, sz, sy, sx, &p, &w);
+                    gather_pw_rt(tm, occ, g.nz, g.ny, g.nx, R, sz, sy, sx, &p, &w);
                     wst[y * g.nx + x] = w;
                     return p;
                 }
@@ -559,7 +565,7 @@ int launch_plane_kernel(const Fft3D& f, const FusedArgs& a, int which, bool skip
         const int ngrid = a.binary_mask ? 2 : 3;
         const int nplanes = skip ? 2 * zr + 1 : f.nz;
         hipLaunchKernelGGL(k_fwd_plane_rt<ODD>, dim3(nplanes, a.nrot), dim3(nthr_plane),
-                           lds_plane, s, a.tm, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, ngrid,
+                           lds_plane, s, a.tm, a.occ, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, ngrid,
                            a.work, f.px, f.py, f.twx, f.twy, f.twfull);
         return launched("k_fwd_plane_rt");
     }

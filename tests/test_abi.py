@@ -32,10 +32,16 @@ def test_argument_errors_need_no_device():
     assert lib.pf_scan(None, 0, 1, 1, 0) == -1
 
 
+def _occ_bytes(nz, ny, nx):
+    """The brick occupancy map: one bit per 8^3 brick, ceil(n / 8) per axis."""
+    nb = ((nz + 7) // 8) * ((ny + 7) // 8) * ((nx + 7) // 8)
+    return 4 * ((nb + 31) // 32)
+
+
 def test_ctx_bytes_model():
     """Closed form of the admission estimate: resident inputs and keys,
-    rotations, and the larger of the scan's work buffers and the transient
-    buffers of pf_set_target (16 bytes per voxel)."""
+    rotations, the brick occupancy map, and the larger of the scan's work
+    buffers and the transient buffers of pf_set_target (16 bytes per voxel)."""
     lib = _lib.load()
     n, b, K = 128, 64, 7416
     V, H = n ** 3, n * n * (n // 2 + 1)
@@ -63,19 +69,19 @@ def test_ctx_bytes_model():
     n2 = 120 + 6
     V2, H2 = n2 ** 3, n2 * n2 * (n2 // 2 + 1)
     assert lib.pf_ctx_bytes(n2, n2, n2, b, _lib.MODES["auto"], 0, 0) == \
-        V2 * 25 + 2 * H2 * 8 + 3 * b * H2 * 8
+        V2 * 25 + 2 * H2 * 8 + _occ_bytes(n2, n2, n2) + 3 * b * H2 * 8
     assert lib.pf_ctx_bytes(n2, n2, n2, b, _lib.MODES["fused_rt"], 0, 0) == \
-        V2 * 25 + 2 * H2 * 8 + 3 * b * H2 * 8
+        V2 * 25 + 2 * H2 * 8 + _occ_bytes(n2, n2, n2) + 3 * b * H2 * 8
     # odd x (two rows per x transform) within the planes' limits: spectra only
     V4, H4 = 125 ** 3, 125 * 125 * 63
     assert lib.pf_ctx_bytes(125, 125, 125, b, _lib.MODES["auto"], 0, 0) == \
-        V4 * 25 + 2 * H4 * 8 + 3 * b * H4 * 8
+        V4 * 25 + 2 * H4 * 8 + _occ_bytes(125, 125, 125) + 3 * b * H4 * 8
     # planes of more than 16,384 voxels (odd and even x): the unfused layout
     for shape in ((135, 135, 135), (200, 200, 200)):
         nz, ny, nx = shape
         V3, H3 = nz * ny * nx, nz * ny * (nx // 2 + 1)
         assert lib.pf_ctx_bytes(nz, ny, nx, b, _lib.MODES["auto"], 0, 0) == \
-            V3 * 25 + 2 * H3 * 8 + 3 * b * (H3 * 8 + V3 * 4)
+            V3 * 25 + 2 * H3 * 8 + _occ_bytes(nz, ny, nx) + 3 * b * (H3 * 8 + V3 * 4)
     assert lib.pf_ctx_bytes(0, 1, 1, 1, 0, 0, 0) == -1
     assert lib.pf_ctx_bytes(8, 8, 8, 1, 0, 0, -1) == -1
 
