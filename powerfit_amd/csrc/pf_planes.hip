@@ -34,7 +34,7 @@ namespace {
 // each multiply.  BIN: W's spectrum gives both A (F1) and B (F2).
 __global__ __launch_bounds__(kRtZThreads) void k_zcorr_rt(
     float2* __restrict__ X, const float2* __restrict__ F1, const float2* __restrict__ F2,
-    RtGeo g, int nrot, int chunk, int bin, int zr, Plan1D pz, const float2* __restrict__ twz_g) {
+    RtGeo g, int nrot, int chunk, int bin, int zr, PlanRt pz, const float2* __restrict__ twz_g) {
     extern __shared__ float2 lds[];
     const int tile = g.nz * kRtTq;
     float2* const S0 = lds;
@@ -116,7 +116,9 @@ bool plane_rt_supported(int nz, int ny, int nx) {
     if ((int64_t)ny * nx > (int64_t)kRtVox * kRtThreads) return false;
     // z kernel: three column tiles and the z twiddles
     if ((size_t)(3 * kRtTq + 1) * nz * sizeof(float2) > kRtLds) return false;
-    return true;
+    // radix plans that fit the packed kernel argument
+    PlanRt p;
+    return plan_rt(g.rl, &p) && plan_rt(ny, &p) && plan_rt(nz, &p);
 }
 
 int prepare_planes_rt() {
@@ -143,11 +145,16 @@ int launch_planes_rt(const Fft3D& f, const FusedArgs& a, int which, hipStream_t 
     if (which == 1) {
         if (zcorr_ct_supported(f.nz)) return launch_zcorr_ct(f, a, skip ? zr : f.nz, s);
         const RtGeo g = make_geo(f.nz, f.ny, f.nx);
+        PlanRt pz;
+        if (!pack_plan(f.pz, &pz)) {
+            set_error("z stage: no packed radix plan for length %d", f.nz);
+            return kErrUnsupported;
+        }
         const int tiles = (g.plane + kRtTq - 1) / kRtTq;
         const size_t lds = (size_t)(3 * kRtTq + 1) * f.nz * sizeof(float2);
         hipLaunchKernelGGL(k_zcorr_rt, dim3(tiles, (a.nrot + a.chunk_z - 1) / a.chunk_z),
                            dim3(kRtZThreads), lds, s, a.work, a.F1, a.F2, g, a.nrot, a.chunk_z,
-                           a.binary_mask ? 1 : 0, skip ? zr : f.nz, f.pz, f.twz);
+                           a.binary_mask ? 1 : 0, skip ? zr : f.nz, pz, f.twz);
         return launched("k_zcorr_rt");
     }
     if (f.nx % 2) return launch_plane_kernel_odd(f, a, which, skip, zr, s);

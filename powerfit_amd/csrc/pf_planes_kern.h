@@ -157,16 +157,44 @@ __device__ __forceinline__ void rt_stage(const float2* a, float2* b, int Rr, int
     }
 }
 
+// A radix plan as a kernel argument: the length, the stage count and the
+// radices in 8 bits each (stage s in bits 8 s .. 8 s + 7).  Four scalar
+// registers, where Plan1D's radix array would hold eighteen for the whole
+// kernel.
+struct PlanRt {
+    int n, nstages;
+    unsigned long long radix;
+};
+
+// Pack a plan; false when it has more than 8 stages or a radix above 255.
+inline bool pack_plan(const Plan1D& p, PlanRt* out) {
+    if (p.nstages > 8) return false;
+    out->n = p.n;
+    out->nstages = p.nstages;
+    out->radix = 0;
+    for (int s = 0; s < p.nstages; ++s) {
+        if (p.radix[s] < 2 || p.radix[s] > 255) return false;
+        out->radix |= (unsigned long long)p.radix[s] << (8 * s);
+    }
+    return true;
+}
+
+// The packed plan of length n (make_plan's factorisation); false if none.
+inline bool plan_rt(int n, PlanRt* out) {
+    Plan1D p;
+    return make_plan(n, &p) && pack_plan(p, out);
+}
+
 // lds_fft (pf_fft.h) over nfft transforms in LDS with the radix of each stage
 // dispatched once per stage; twiddles are read from LDS.  Returns the buffer
 // holding the result; ends with a barrier.
 template <int DIR>
-__device__ float2* lds_fft_rt(float2* a, float2* b, const Plan1D& p, const float2* tw, int nfft,
+__device__ float2* lds_fft_rt(float2* a, float2* b, const PlanRt& p, const float2* tw, int nfft,
                               int fs, int es) {
     const int n = p.n;
     int Ns = 1;
     for (int s = 0; s < p.nstages; ++s) {
-        const int R = p.radix[s];
+        const int R = (int)((p.radix >> (8 * s)) & 255ull);
         switch (R) {
             case 2: rt_stage<2, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
             case 3: rt_stage<3, DIR>(a, b, R, Ns, n, tw, nfft, fs, es); break;
@@ -190,7 +218,7 @@ __device__ float2* lds_fft_rt(float2* a, float2* b, const Plan1D& p, const float
 // transform f at a[f + e st]; otherwise at a[f st + e] (ODDROWS: the odd row
 // lengths).
 template <int DIR, bool COLS, bool ODDROWS = false>
-__device__ __forceinline__ float2* tile_fft(float2* a, float2* b, const Plan1D& p,
+__device__ __forceinline__ float2* tile_fft(float2* a, float2* b, const PlanRt& p,
                                             const float2* tw, int nfft, int st) {
     float2* out;
     if (ct_dispatch<DIR, COLS, ODDROWS>(p.n, a, b, nfft, st, tw, &out)) return out;
@@ -261,7 +289,7 @@ template <bool ODD>
 __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
     const float2* __restrict__ tm, const uint32_t* __restrict__ occ,
     const double* __restrict__ rots, RtGeo g, int r, int zr,
-    int zlo, int ngrid, float2* __restrict__ out, Plan1D px, Plan1D py,
+    int zlo, int ngrid, float2* __restrict__ out, PlanRt px, PlanRt py,
     const float2* __restrict__ twx_g, const float2* __restrict__ twy_g,
     const float2* __restrict__ twf_g) {
     extern __shared__ float2 lds[];
@@ -399,7 +427,7 @@ template <bool ODD>
 __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
     const float2* __restrict__ X, const uint8_t* __restrict__ lmask,
     unsigned long long* __restrict__ keys, RtGeo g, int nrot, int chunk, int64_t first,
-    double norm, Plan1D px, Plan1D py, const float2* __restrict__ twx_g,
+    double norm, PlanRt px, PlanRt py, const float2* __restrict__ twx_g,
     const float2* __restrict__ twy_g, const float2* __restrict__ twf_g) {
     extern __shared__ float2 lds[];
     float2* const A = lds;
@@ -500,20 +528,26 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
             // even nx: row y holds 2 M reals (voxel u at float u); odd nx: the
             // packed element e holds rows 2p and 2p + 1 as its real and
             // imaginary parts
+            // One unrolled slot loop per grid, each straight-line in held[]
+            // (a grid test inside the slot loop makes the compiler copy the
+            // whole array at every slot).  Slots without a voxel read float 0
+            // and hold values nothing reads (their mask bit is clear).
             const float* zr = reinterpret_cast<const float*>(Z);
+            if (gi == 2) {
 #pragma unroll
-            for (int j = 0; j < kRtVox; ++j) {
-                const int fi = slot_fi(j);
-                if (fi < 0) continue;
-                const float val = zr[fi];
-                if (gi == 2) {
-                    held[j] = val;
-                } else if (gi == 1) {
-                    const double ad = (double)val;
+                for (int j = 0; j < kRtVox; ++j) held[j] = zr[max(slot_fi(j), 0)];
+            } else if (gi == 1) {
+#pragma unroll
+                for (int j = 0; j < kRtVox; ++j) {
+                    const double ad = (double)zr[max(slot_fi(j), 0)];
                     held[j] = __builtin_amdgcn_rsqf((float)fma(-ad, ad, (double)held[j] * nv));
-                } else if ((mbits >> j) & 1u) {
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < kRtVox; ++j) {
+                    if (!((mbits >> j) & 1u)) continue;
                     // only S > 0 can beat the initial (0, 0); NaN never wins
-                    const float sv = val * held[j];
+                    const float sv = zr[slot_fi(j)] * held[j];
                     if (sv > 0.f)
                         atomicMax(keys + vz + slot_vox(j), lcc_key(sv, (uint32_t)(first + b)));
                 }
@@ -557,6 +591,11 @@ template <bool ODD>
 int launch_plane_kernel(const Fft3D& f, const FusedArgs& a, int which, bool skip, int zr,
                         hipStream_t s) {
     const RtGeo g = make_geo(f.nz, f.ny, f.nx);
+    PlanRt px, py;
+    if (!pack_plan(f.px, &px) || !pack_plan(f.py, &py)) {
+        set_error("plane kernels: no packed radix plan for (%d, %d)", f.ny, f.nx);
+        return kErrUnsupported;
+    }
     const size_t lds_plane = plane_lds_bytes(g);
     const int nthr_plane =
         (2 * lds_plane <= kCuLds && (int64_t)f.ny * f.nx <= (int64_t)kRtVox * 512) ? 512
@@ -566,12 +605,12 @@ int launch_plane_kernel(const Fft3D& f, const FusedArgs& a, int which, bool skip
         const int nplanes = skip ? 2 * zr + 1 : f.nz;
         hipLaunchKernelGGL(k_fwd_plane_rt<ODD>, dim3(nplanes, a.nrot), dim3(nthr_plane),
                            lds_plane, s, a.tm, a.occ, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, ngrid,
-                           a.work, f.px, f.py, f.twx, f.twy, f.twfull);
+                           a.work, px, py, f.twx, f.twy, f.twfull);
         return launched("k_fwd_plane_rt");
     }
     hipLaunchKernelGGL(k_inv_plane_rt<ODD>, dim3(f.nz, (a.nrot + a.chunk_lcc - 1) / a.chunk_lcc),
                        dim3(nthr_plane), lds_plane, s, a.work, a.lcc_mask, a.keys, g, a.nrot,
-                       a.chunk_lcc, a.first, a.norm, f.px, f.py, f.twx, f.twy, f.twfull);
+                       a.chunk_lcc, a.first, a.norm, px, py, f.twx, f.twy, f.twfull);
     return launched("k_inv_plane_rt");
 }
 
