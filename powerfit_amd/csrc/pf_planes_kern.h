@@ -284,12 +284,13 @@ __host__ __device__ inline size_t plane_lds_bytes(const RtGeo& g) {
     return sizeof(float2) * ((size_t)2 * g.buf + g.rl + g.ny + g.nx);
 }
 
-// Forward stage.  Planes zlo, zlo + 1, ... (mod nz) of the support.
-template <bool ODD>
+// Forward stage for grid GI (0: P, gathering and parking W; 1: W; 2: W^2),
+// one launch per grid.  Planes zlo, zlo + 1, ... (mod nz) of the support.
+template <bool ODD, int GI>
 __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
     const float2* __restrict__ tm, const uint32_t* __restrict__ occ,
     const double* __restrict__ rots, RtGeo g, int r, int zr,
-    int zlo, int ngrid, float2* __restrict__ out, PlanRt px, PlanRt py,
+    int zlo, float2* __restrict__ out, PlanRt px, PlanRt py,
     const float2* __restrict__ twx_g, const float2* __restrict__ twy_g,
     const float2* __restrict__ twf_g) {
     extern __shared__ float2 lds[];
@@ -306,17 +307,20 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
     // gathered voxels: inside the rotation sphere and the support ball
     const int rg = r < zr ? r : zr, rg2 = rg * rg;
     double R[9];
+    if constexpr (GI == 0) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
+        for (int i = 0; i < 9; ++i) R[i] = rots[9 * b + i];
+    }
     float2* dst = out + (int64_t)b * 3 * g.H + (int64_t)iz * g.plane;
     // The mask values gathered with P are parked in this plane's region of
-    // grid 2 (written last, by this block only, after every read of it; each
-    // thread reads back only what it wrote), voxel (y, x) at float y nx + x, for
-    // the W (and W^2) grids.
+    // grid 2 (read back by the W and W^2 launches; grid 2 is written last, by
+    // the block of the same plane, after all its reads), voxel (y, x) at float
+    // y nx + x.
     float2* const wst2 = dst + 2 * g.H;
     float* const wst = reinterpret_cast<float*>(wst2);
     const int rows = g.nr * g.rl;
-    for (int gi = 0; gi < ngrid; ++gi) {
+    {
+        constexpr int gi = GI;
         // packed rows of the x transform, gathered for grid 0 (P, with W
         // parked) and read back for W and W^2
         if constexpr (!ODD) {
@@ -601,12 +605,23 @@ int launch_plane_kernel(const Fft3D& f, const FusedArgs& a, int which, bool skip
         (2 * lds_plane <= kCuLds && (int64_t)f.ny * f.nx <= (int64_t)kRtVox * 512) ? 512
                                                                                    : kRtThreads;
     if (which == 0) {
-        const int ngrid = a.binary_mask ? 2 : 3;
         const int nplanes = skip ? 2 * zr + 1 : f.nz;
-        hipLaunchKernelGGL(k_fwd_plane_rt<ODD>, dim3(nplanes, a.nrot), dim3(nthr_plane),
-                           lds_plane, s, a.tm, a.occ, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, ngrid,
-                           a.work, px, py, f.twx, f.twy, f.twfull);
-        return launched("k_fwd_plane_rt");
+        const dim3 grid(nplanes, a.nrot);
+        hipLaunchKernelGGL((k_fwd_plane_rt<ODD, 0>), grid, dim3(nthr_plane), lds_plane, s, a.tm,
+                           a.occ, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, a.work, px, py, f.twx,
+                           f.twy, f.twfull);
+        PF_CHECK_STATUS(launched("k_fwd_plane_rt"));
+        hipLaunchKernelGGL((k_fwd_plane_rt<ODD, 1>), grid, dim3(nthr_plane), lds_plane, s, a.tm,
+                           a.occ, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, a.work, px, py, f.twx,
+                           f.twy, f.twfull);
+        PF_CHECK_STATUS(launched("k_fwd_plane_rt"));
+        if (!a.binary_mask) {
+            hipLaunchKernelGGL((k_fwd_plane_rt<ODD, 2>), grid, dim3(nthr_plane), lds_plane, s,
+                               a.tm, a.occ, a.rots, g, a.rmax, a.zr, skip ? -zr : 0, a.work, px,
+                               py, f.twx, f.twy, f.twfull);
+            PF_CHECK_STATUS(launched("k_fwd_plane_rt"));
+        }
+        return kOk;
     }
     hipLaunchKernelGGL(k_inv_plane_rt<ODD>, dim3(f.nz, (a.nrot + a.chunk_lcc - 1) / a.chunk_lcc),
                        dim3(nthr_plane), lds_plane, s, a.work, a.lcc_mask, a.keys, g, a.nrot,
@@ -616,7 +631,9 @@ int launch_plane_kernel(const Fft3D& f, const FusedArgs& a, int which, bool skip
 
 template <bool ODD>
 int prepare_plane_kernels() {
-    PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt<ODD>, kRtLds));
+    PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt<ODD, 0>, kRtLds));
+    PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt<ODD, 1>, kRtLds));
+    PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt<ODD, 2>, kRtLds));
     PF_CHECK_STATUS(allow_lds((const void*)k_inv_plane_rt<ODD>, kRtLds));
     return kOk;
 }
