@@ -624,14 +624,22 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
                 // B, then 1/sqrt(var), of voxels x = 2m, 2m + 1 in the B plane's region
                 float2* slot = const_cast<float2*>(X) + ((int64_t)b * 3 + 2) * G::H +
                                (int64_t)iz * G::PLANE + row * M + tx;
+                if (g == 2) {
 #pragma unroll
-                for (int e = 0; e < EX; ++e) {
-                    float2 held = g == 2 ? vr[e] : slot[TX * e];
-                    if (g == 1) {
+                    for (int e = 0; e < EX; ++e) slot[TX * e] = vr[e];
+                } else if (g == 1) {
+#pragma unroll
+                    for (int e = 0; e < EX; ++e) {
                         // N V B - A^2 in FP64 (exact products), then 1/sqrt in FP32
+                        float2 held = slot[TX * e];
                         held.x = rsq_var(vr[e].x, held.x, nv);
                         held.y = rsq_var(vr[e].y, held.y, nv);
-                    } else if (g == 0) {
+                        slot[TX * e] = held;
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < EX; ++e) {
+                        const float2 held = slot[TX * e];
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
                             if (!((mbits >> (2 * (it * EX + e) + h)) & MaskT(1))) continue;
@@ -644,23 +652,33 @@ __global__ __launch_bounds__(NTHR) void k_inv_plane(typename std::conditional<IS
                             }
                         }
                     }
-                    if (g != 0) slot[TX * e] = held;
                 }
             } else if (active) {
+                // one straight-line loop per grid (a grid test per element
+                // makes the compiler copy held values between branches)
+                if (g == 2) {
 #pragma unroll
-                for (int e = 0; e < EX; ++e) {
+                    for (int e = 0; e < EX; ++e) {
+                        isd[ISDG ? 0 : it][ISDG ? 0 : e][0] = vr[e].x;
+                        isd[ISDG ? 0 : it][ISDG ? 0 : e][1] = vr[e].y;
+                    }
+                } else if (g == 1) {
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const float val = h ? vr[e].y : vr[e].x;
-                        float& is = isd[ISDG ? 0 : it][ISDG ? 0 : e][h];
-                        if (g == 2) {
-                            is = val;
-                        } else if (g == 1) {
-                            // N V B - A^2 in FP64 (exact products), then 1/sqrt in FP32
-                            is = rsq_var(val, is, nv);
-                        } else if ((mbits >> (2 * (it * EX + e) + h)) & MaskT(1)) {
+                    for (int e = 0; e < EX; ++e) {
+                        // N V B - A^2 in FP64 (exact products), then 1/sqrt in FP32
+                        float* is = isd[ISDG ? 0 : it][ISDG ? 0 : e];
+                        is[0] = rsq_var(vr[e].x, is[0], nv);
+                        is[1] = rsq_var(vr[e].y, is[1], nv);
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < EX; ++e) {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            if (!((mbits >> (2 * (it * EX + e) + h)) & MaskT(1))) continue;
                             // only S > 0 can beat the initial (0, 0); NaN never wins
-                            const float sv = val * is;
+                            const float sv = (h ? vr[e].y : vr[e].x) *
+                                             isd[ISDG ? 0 : it][ISDG ? 0 : e][h];
                             if (sv > 0.f) {
                                 const int64_t vox =
                                     ((int64_t)iz * N + row) * N + 2 * (tx + TX * e) + h;
