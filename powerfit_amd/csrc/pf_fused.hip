@@ -1203,6 +1203,15 @@ struct Shape80 : Shape<320, 16, 16, 320, 12, 24> {
 };
 // 128: one-plane K1 in 512-thread blocks, one launch per grid (7.3 against
 // 8.4 us per rotation for the two-plane K1)
+// 48, 96: one K1 launch per grid for weighted masks (the mask values parked;
+// per rotation with a core-weighted mask 0.77 -> 0.64 us at 48^3, 6.4 -> 4.5 us
+// at 96^3; at 64^3 and 144^3 the single launch stays faster)
+struct Shape48 : Shape<192, 32, 12, 96, 16, 6> {
+    static constexpr bool STASHW = true;
+};
+struct Shape96 : Shape<768, 32, 12, 384, 16, 24> {
+    static constexpr bool STASHW = true;
+};
 struct Shape128 : Shape<512, 32, 16, 1024, 16, 8> {
     static constexpr bool PAIR1 = false, STASHW = true, HOIST3 = true;
 };
@@ -1323,11 +1332,11 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
         return kErrArg;
     }
     switch (a.n) {
-        case 48: return run_fused<48, Shape<192, 32, 12, 96, 16, 6>>(a, which, s);
+        case 48: return run_fused<48, Shape48>(a, which, s);
         case 64: return run_fused<64, Shape<256, 32, 8, 256, 16, 4>>(a, which, s);
         case 72: return run_fused<72, Shape72>(a, which, s);
         case 80: return run_fused<80, Shape80>(a, which, s);
-        case 96: return run_fused<96, Shape<768, 32, 12, 384, 16, 24>>(a, which, s);
+        case 96: return run_fused<96, Shape96>(a, which, s);
         case 128: return run_fused<128, Shape128>(a, which, s);
         case 144: return run_fused<144, Shape144>(a, which, s);
         case 160: return run_fused<160, Shape160>(a, which, s);
