@@ -1176,7 +1176,8 @@ constexpr bool zsplit_ok() {
 
 // Launch shape of one fused size.  K1: NTHR1 threads, element bounds EMAX (rows,
 // and the two-plane kernels) and EMAXC1 (one-plane column pass), mask values
-// kept in global memory and one launch per grid when STASHW.  K2: TQ columns per
+// kept in global memory and one launch per grid when STASHW (weighted masks)
+// or STASHB (0/1 masks without the two-plane kernel).  K2: TQ columns per
 // block (TQ2 the alternative width), element bounds EMAXZ / EMAXZ2 of the
 // default and the alternative radix plan (FusedArgs::zsched: bit 0 the radix
 // plan, bit 1 the tile width).  K3: NTHR3 threads, element bounds
@@ -1189,6 +1190,7 @@ struct Shape {
                          EMAXZ2 = EZ2_, EMAX3 = E_, EMAXC3 = E_, EMAXC1 = E_;
     static constexpr int TQ2 = TQ_ == 32 ? 16 : 32;   // alternative z tile width
     static constexpr bool XPF3 = true, ISDG3 = false, RECOMP3 = false, STASHW = false;
+    static constexpr bool STASHB = STASHW;
     static constexpr bool HOIST3 = false;   // two-plane K3: copy offsets held in registers
     static constexpr bool PAIR1 = true, PAIR3 = true;   // two-plane K1 / K3 where they fit
 };
@@ -1196,24 +1198,32 @@ struct Shape {
 // 72: two-plane K3 in 576-thread blocks (4.3 against 5.6 us per rotation);
 // K1 in 768-thread blocks, two-plane for 0/1 masks (2.6 against 3.1 us)
 struct Shape72 : Shape<768, 32, 12, 576, 16, 6> {
-    static constexpr bool STASHW = true;
+    static constexpr bool STASHW = true, STASHB = true;
 };
 struct Shape80 : Shape<320, 16, 16, 320, 12, 24> {
-    static constexpr bool RECOMP3 = true, STASHW = true;
+    static constexpr bool RECOMP3 = true, STASHW = true, STASHB = true;
 };
-// 128: one-plane K1 in 512-thread blocks, one launch per grid (7.3 against
-// 8.4 us per rotation for the two-plane K1)
-// 48, 96: one K1 launch per grid for weighted masks (the mask values parked;
-// per rotation with a core-weighted mask 0.77 -> 0.64 us at 48^3, 6.4 -> 4.5 us
-// at 96^3; at 64^3 and 144^3 the single launch stays faster)
+// 48: one K1 launch per grid for weighted masks (the mask values parked; 0.77
+// -> 0.64 us per rotation with a core-weighted mask); two-plane for 0/1 masks
 struct Shape48 : Shape<192, 32, 12, 96, 16, 6> {
     static constexpr bool STASHW = true;
 };
-struct Shape96 : Shape<768, 32, 12, 384, 16, 24> {
-    static constexpr bool STASHW = true;
+// 64: 0/1 masks one-plane, one launch per grid (0.60 -> 0.43 us per rotation
+// against the two-plane K1); weighted masks one launch for all grids (faster
+// there)
+struct Shape64 : Shape<256, 32, 8, 256, 16, 4> {
+    static constexpr bool PAIR1 = false, STASHB = true;
 };
+// 96: one-plane K1, one launch per grid for both mask kinds (per rotation 6.4
+// -> 4.5 us core-weighted, 4.1 -> 3.3 us for 0/1 masks against the two-plane
+// K1)
+struct Shape96 : Shape<768, 32, 12, 384, 16, 24> {
+    static constexpr bool STASHW = true, STASHB = true, PAIR1 = false;
+};
+// 128: one-plane K1 in 512-thread blocks, one launch per grid (7.3 against
+// 8.4 us per rotation for the two-plane K1)
 struct Shape128 : Shape<512, 32, 16, 1024, 16, 8> {
-    static constexpr bool PAIR1 = false, STASHW = true, HOIST3 = true;
+    static constexpr bool PAIR1 = false, STASHW = true, STASHB = true, HOIST3 = true;
 };
 struct Shape144 : Shape<864, 16, 12, 864, 16, 6> {
     static constexpr bool RECOMP3 = true;
@@ -1222,12 +1232,12 @@ struct Shape144 : Shape<864, 16, 12, 864, 16, 6> {
 // 20 elements per thread and keeps the variance in global memory
 struct Shape160 : Shape<704, 16, 10, 640, 20, 20> {
     static constexpr int EMAX3 = 10, EMAXC3 = 20;
-    static constexpr bool XPF3 = false, ISDG3 = true, STASHW = true;
+    static constexpr bool XPF3 = false, ISDG3 = true, STASHW = true, STASHB = true;
 };
 // 192: K3 runs its 96 columns in two sweeps of 48 (12 elements per thread)
 struct Shape192 : Shape<784, 16, 12, 768, 24, 24> {
     static constexpr int EMAX3 = 12, EMAXC3 = 12;
-    static constexpr bool XPF3 = false, ISDG3 = true, STASHW = true;
+    static constexpr bool XPF3 = false, ISDG3 = true, STASHW = true, STASHB = true;
 };
 
 template <int N, class SH>
@@ -1247,7 +1257,7 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
             launch_fwd_pair<N, NTHR1, EMAX>(dim3(nplanes, a.nrot), s, a.tm, a.rots, a.rmax, a.zr,
                                             zlo, a.work, tw.M, tw.N, a.occ);
         else if (a.binary_mask)
-            launch_fwd_plane<N, NTHR1, true, EMAX, SH::STASHW, SH::EMAXC1>(
+            launch_fwd_plane<N, NTHR1, true, EMAX, SH::STASHB, SH::EMAXC1>(
                 dim3(nplanes, a.nrot), s, a.tm, a.rots, a.rmax, a.zr, zlo, a.work, tw.M, tw.N,
                 a.occ);
         else
@@ -1333,7 +1343,7 @@ int launch_fused(const FusedArgs& a, int which, hipStream_t s) {
     }
     switch (a.n) {
         case 48: return run_fused<48, Shape48>(a, which, s);
-        case 64: return run_fused<64, Shape<256, 32, 8, 256, 16, 4>>(a, which, s);
+        case 64: return run_fused<64, Shape64>(a, which, s);
         case 72: return run_fused<72, Shape72>(a, which, s);
         case 80: return run_fused<80, Shape80>(a, which, s);
         case 96: return run_fused<96, Shape96>(a, which, s);
