@@ -1225,8 +1225,10 @@ struct Shape96 : Shape<768, 32, 12, 384, 16, 24> {
 struct Shape128 : Shape<512, 32, 16, 1024, 16, 8> {
     static constexpr bool PAIR1 = false, STASHW = true, STASHB = true, HOIST3 = true;
 };
+// 144: the next plane is not prefetched during the row phase (K3 23.4 ->
+// 22.1 us per rotation: the prefetch registers spill at 864 threads)
 struct Shape144 : Shape<864, 16, 12, 864, 16, 6> {
-    static constexpr bool RECOMP3 = true;
+    static constexpr bool RECOMP3 = true, XPF3 = false;
 };
 // 160 = 2^5 * 5 has no 16-element plan; K3 sweeps its 80 columns once with
 // 20 elements per thread and keeps the variance in global memory
