@@ -6,12 +6,13 @@
 // lengths and radix plans taken at run time: every transform is an LDS
 // Stockham pass (lds_fft) over a whole z-plane or a tile of z-columns.
 //
-//   k_fwd_plane_rt  one block per (z-plane, rotation): gathers the rotated
-//                   template (trilinear) and mask (nearest) of the plane row
-//                   by row, R2C along x (half-length complex FFT + post-
-//                   processing), C2C along y, and writes the xy-spectra of P,
-//                   W (and W^2 for weighted masks) in numpy's [z][ky][kx]
-//                   layout.
+//   k_fwd_plane_rt  one block per (z-plane, rotation), one launch per grid:
+//                   the P launch gathers the rotated template (trilinear) and
+//                   mask (nearest) of the plane row by row and parks the mask
+//                   values, the W (and W^2 for weighted masks) launches read
+//                   them back; each then runs R2C along x (half-length complex
+//                   FFT + post-processing), C2C along y, and writes its
+//                   xy-spectrum in numpy's [z][ky][kx] layout.
 //   k_zcorr_rt      one block per (tile of (ky,kx) columns, rotation chunk):
 //                   forward FFT along z, conj(X) F1 / conj(X) F2 with the F
 //                   tile in LDS, inverse FFT along z.
