@@ -534,8 +534,9 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
             // imaginary parts
             // One unrolled slot loop per grid, each straight-line in held[]
             // (a grid test inside the slot loop makes the compiler copy the
-            // whole array at every slot).  Slots without a voxel read float 0
-            // and hold values nothing reads (their mask bit is clear).
+            // whole array at every slot).  Slots without a voxel read the
+            // plane's first value instead and hold values nothing reads (their
+            // mask bit is clear).
             const float* zr = reinterpret_cast<const float*>(Z);
             if (gi == 2) {
 #pragma unroll
