@@ -16,7 +16,7 @@ namespace {
 
 constexpr int kRtThreads = 1024;         // plane kernels: largest block
 constexpr size_t kCuLds = 160 * 1024;    // LDS of one compute unit
-constexpr int kRtVox = 16;               // voxels of a plane owned per thread (K3)
+constexpr int kRtVox = 16;               // voxels of a plane owned per thread (K3), at most
 constexpr int kRtZThreads = 256;         // z kernel
 constexpr int kRtTq = 16;                // z columns per block
 constexpr size_t kRtLds = 150 * 1024;    // dynamic LDS a plane kernel may take
@@ -422,12 +422,12 @@ __global__ __launch_bounds__(kRtThreads) void k_fwd_plane_rt(
     }
 }
 
-// Inverse stage and LCC.  Each thread owns kRtVox voxel slots j of the plane:
+// Inverse stage and LCC.  Each thread owns VOX voxel slots j of the plane:
 // even nx, voxel u = threadIdx.x + j blockDim.x (row-major (y, x)); odd nx,
 // packed element e = threadIdx.x + (j / 2) blockDim.x of the x-transformed
 // rows, voxel (2 (e / nx) + j % 2, e % nx).  It keeps B, then 1 / sqrt(N V B -
-// A^2), of them in registers.
-template <bool ODD>
+// A^2), of them in registers (VOX 12 where the plane allows: fewer registers).
+template <bool ODD, int VOX>
 __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
     const float2* __restrict__ X, const uint8_t* __restrict__ lmask,
     unsigned long long* __restrict__ keys, RtGeo g, int nrot, int chunk, int64_t first,
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
     const int nvox = g.ny * g.nx, npk = g.nr * g.nx;
     const double nv = norm * ((double)g.nz * g.ny * g.nx);   // N V (exact)
     const int64_t vz = (int64_t)iz * nvox;
-    static_assert(kRtVox <= 32 && kRtVox % 2 == 0, "mask bits of the owned voxels fit a 32-bit word");
+    static_assert(VOX <= 32 && VOX % 2 == 0, "mask bits of the owned voxels fit a 32-bit word");
     // slot j: float index of its value in the x-transformed plane (-1: none)
     auto slot_fi = [&](int j) -> int {
         if constexpr (!ODD) {
@@ -466,13 +466,13 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
     };
     uint32_t mbits = 0;
 #pragma unroll
-    for (int j = 0; j < kRtVox; ++j) {
+    for (int j = 0; j < VOX; ++j) {
         const int fi = slot_fi(j);
         if (fi < 0) continue;
         const int v = slot_vox(j);
         if (v < nvox && lmask[vz + v]) mbits |= 1u << j;
     }
-    float held[kRtVox];
+    float held[VOX];
     for (int b = b0; b < b1; ++b) {
         for (int gi = 2; gi >= 0; --gi) {
             const float2* src = X + ((int64_t)b * 3 + gi) * g.H + (int64_t)iz * g.plane;
@@ -540,16 +540,16 @@ __global__ __launch_bounds__(kRtThreads) void k_inv_plane_rt(
             const float* zr = reinterpret_cast<const float*>(Z);
             if (gi == 2) {
 #pragma unroll
-                for (int j = 0; j < kRtVox; ++j) held[j] = zr[max(slot_fi(j), 0)];
+                for (int j = 0; j < VOX; ++j) held[j] = zr[max(slot_fi(j), 0)];
             } else if (gi == 1) {
 #pragma unroll
-                for (int j = 0; j < kRtVox; ++j) {
+                for (int j = 0; j < VOX; ++j) {
                     const double ad = (double)zr[max(slot_fi(j), 0)];
                     held[j] = __builtin_amdgcn_rsqf((float)fma(-ad, ad, (double)held[j] * nv));
                 }
             } else {
 #pragma unroll
-                for (int j = 0; j < kRtVox; ++j) {
+                for (int j = 0; j < VOX; ++j) {
                     if (!((mbits >> j) & 1u)) continue;
                     // only S > 0 can beat the initial (0, 0); NaN never wins
                     const float sv = zr[slot_fi(j)] * held[j];
@@ -624,9 +624,15 @@ int launch_plane_kernel(const Fft3D& f, const FusedArgs& a, int which, bool skip
         }
         return kOk;
     }
-    hipLaunchKernelGGL(k_inv_plane_rt<ODD>, dim3(f.nz, (a.nrot + a.chunk_lcc - 1) / a.chunk_lcc),
-                       dim3(nthr_plane), lds_plane, s, a.work, a.lcc_mask, a.keys, g, a.nrot,
-                       a.chunk_lcc, a.first, a.norm, px, py, f.twx, f.twy, f.twfull);
+    const dim3 grid3(f.nz, (a.nrot + a.chunk_lcc - 1) / a.chunk_lcc);
+    if ((int64_t)f.ny * f.nx <= (int64_t)12 * nthr_plane)
+        hipLaunchKernelGGL((k_inv_plane_rt<ODD, 12>), grid3, dim3(nthr_plane), lds_plane, s,
+                           a.work, a.lcc_mask, a.keys, g, a.nrot, a.chunk_lcc, a.first, a.norm,
+                           px, py, f.twx, f.twy, f.twfull);
+    else
+        hipLaunchKernelGGL((k_inv_plane_rt<ODD, kRtVox>), grid3, dim3(nthr_plane), lds_plane, s,
+                           a.work, a.lcc_mask, a.keys, g, a.nrot, a.chunk_lcc, a.first, a.norm,
+                           px, py, f.twx, f.twy, f.twfull);
     return launched("k_inv_plane_rt");
 }
 
@@ -635,7 +641,8 @@ int prepare_plane_kernels() {
     PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt<ODD, 0>, kRtLds));
     PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt<ODD, 1>, kRtLds));
     PF_CHECK_STATUS(allow_lds((const void*)k_fwd_plane_rt<ODD, 2>, kRtLds));
-    PF_CHECK_STATUS(allow_lds((const void*)k_inv_plane_rt<ODD>, kRtLds));
+    PF_CHECK_STATUS(allow_lds((const void*)k_inv_plane_rt<ODD, 12>, kRtLds));
+    PF_CHECK_STATUS(allow_lds((const void*)k_inv_plane_rt<ODD, kRtVox>, kRtLds));
     return kOk;
 }
 
