@@ -205,8 +205,11 @@ int pf_ctx_set_tuning(pf_ctx *ctx, int chunk_z, int chunk_lcc);
  * element count per thread and radix sequence), bit 1 the width of the
  * (ky, kx) column tile per block (the default, or an alternative width); all
  * four are compiled for every fused size, 0 being that size's measured best.
- * Picked by the autotuner; the radix plan changes results by FP32 rounding of
- * the FFTs, the tile width not at all.  No counterpart in the reference. */
+ * The run-time planned z stage (PF_MODE_FUSED_RT) reads bit 0 only: its
+ * alternative register plan (8 slots per thread instead of 16, more threads
+ * per column) where the z length has one, else the default.  Picked by the
+ * autotuner; the radix plan changes results by FP32 rounding of the FFTs, the
+ * tile width not at all.  No counterpart in the reference. */
 #define PF_Z_SCHEDULES 4
 int pf_ctx_set_schedule(pf_ctx *ctx, int z_schedule);
 int pf_ctx_profile(pf_ctx *ctx, int enable);

@@ -3,7 +3,8 @@
 Times short scans of the correlator's own inputs over candidate rotation-batch
 sizes and per-block rotation chunks of the fused kernels, then, at the best
 of those, the other schedules of the fused z stage (alternative FFT radix
-plan, alternative column tile width, both), and keeps the fastest.  Results are cached per (device, grid, mask kind, mode, precision) in a JSON
+plan, alternative column tile width, both; for the run-time planned z stage
+its alternative register plan), and keeps the fastest.  Results are cached per (device, grid, mask kind, mode, precision) in a JSON
 file (``$POWERFIT_AMD_TUNE`` or ``~/.cache/powerfit_amd/tune.json``).  Batch
 and chunk sizes do not change the result (every rotation's LCC map is computed
 identically and merged with an order-independent max); the FFT schedule
@@ -97,12 +98,17 @@ def autotune(corr, batches=DEFAULT_BATCHES, chunks=DEFAULT_CHUNKS, nrot=None, us
                       flush=True)
     best = max(trials, key=lambda t: t["rate"])
     best["z_schedule"] = 0
-    # split precision runs the fused z stage's default schedule only
-    if (schedules and corr.fused_available and corr.mode in ("auto", "fused")
+    # split precision runs the fused z stage's default schedule only; the
+    # run-time planned z stage reads bit 0 only (its register plan)
+    if (corr.fused_available and corr.mode in ("auto", "fused")
             and getattr(corr, "precision", "fp32") != "split"):
-        for zs in schedules:
-            if zs == 0:
-                continue
+        z_trials = [zs for zs in schedules if zs != 0]
+    elif corr.fused_rt_available and corr.mode in ("auto", "fused_rt"):
+        z_trials = [zs for zs in schedules if zs == 1]
+    else:
+        z_trials = []
+    if z_trials:
+        for zs in z_trials:
             rate = _time(corr, best["batch"], (best["chunk_z"], best["chunk_lcc"]), nrot,
                          z_schedule=zs)
             trials.append({"batch": best["batch"], "chunk_z": best["chunk_z"],

@@ -290,6 +290,13 @@ constexpr FlexPlan flex_plan(int L, int emax, int tmax, bool pow2) {
     return best;
 }
 
+// Register slots per thread of the run-time planned z stage's transforms:
+// the default plan, and an alternative with at most half as many (more
+// threads per column, fewer registers per thread) for lengths that have one.
+constexpr int kFlexZSlots = 16;
+constexpr int kFlexZSlotsAlt = 8;
+constexpr bool flex_zalt_ok(int L) { return flex_plan(L, kFlexZSlotsAlt, 32, true).ns >= 1; }
+
 template <int L, int EMAX, int TMAX, bool REV, bool POW2 = false>
 struct Flex {
     static constexpr FlexPlan F = flex_plan(L, EMAX, TMAX, POW2);

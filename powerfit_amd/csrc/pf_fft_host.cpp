@@ -177,15 +177,15 @@ extern "C" int pf_host_ct_plan(int n, int* radix, int max) {
 }
 
 // Host run of the register FFTs with per-stage butterfly counts
-// (pf_ct_fft.h Flex, 16 register slots, up to 32 threads per transform, a
+// (pf_ct_fft.h Flex, EMAX register slots, up to 32 threads per transform, a
 // power of two of them: the z stage's parameters): the T threads' registers are filled in the input
 // pattern, every stage is emulated thread by thread, and the outputs are read
 // from the output pattern.  info[0..3] = T, E, stages, and 1 when the output
 // pattern of the forward plan equals the input pattern of the reversed plan
 // (and the reversed plan's output the forward's input) slot for slot.
-template <int L, bool REV, int DIR>
+template <int L, int EMAX, bool REV, int DIR>
 static void flex_host_run(const float* in, float* out) {
-    using FX = pf::Flex<L, 16, 32, REV, true>;
+    using FX = pf::Flex<L, EMAX, 32, REV, true>;
     constexpr int T = FX::T, E = FX::E;
     std::vector<float2> tw(L), ex(L);
     for (int m = 0; m < L; ++m) {
@@ -199,7 +199,7 @@ static void flex_host_run(const float* in, float* out) {
             const int p = FX::in_pos(t, s);
             if (p >= 0) v[t][s] = pf::c2(in[2 * p], in[2 * p + 1]);
         }
-    pf::FlexStage<L, 16, 32, REV, DIR, 0, true>::emulate(v, ex.data(), tw.data());
+    pf::FlexStage<L, EMAX, 32, REV, DIR, 0, true>::emulate(v, ex.data(), tw.data());
     for (int t = 0; t < T; ++t)
         for (int s = 0; s < FX::out_slots(); ++s) {
             const int p = FX::out_pos(t, s);
@@ -210,12 +210,14 @@ static void flex_host_run(const float* in, float* out) {
         }
 }
 
-template <int L>
+template <int L, int EMAX>
 static int flex_host(int dir, int rev, const float* in, float* out, int* info) {
-    using F = pf::Flex<L, 16, 32, false, true>;
-    using B = pf::Flex<L, 16, 32, true, true>;
-    if (dir < 0) rev ? flex_host_run<L, true, -1>(in, out) : flex_host_run<L, false, -1>(in, out);
-    else rev ? flex_host_run<L, true, 1>(in, out) : flex_host_run<L, false, 1>(in, out);
+    using F = pf::Flex<L, EMAX, 32, false, true>;
+    using B = pf::Flex<L, EMAX, 32, true, true>;
+    if (dir < 0)
+        rev ? flex_host_run<L, EMAX, true, -1>(in, out) : flex_host_run<L, EMAX, false, -1>(in, out);
+    else
+        rev ? flex_host_run<L, EMAX, true, 1>(in, out) : flex_host_run<L, EMAX, false, 1>(in, out);
     int same = F::out_slots() == B::in_slots() && B::out_slots() == F::in_slots();
     for (int t = 0; t < F::T && same; ++t) {
         for (int s = 0; s < F::out_slots(); ++s) same &= F::out_pos(t, s) == B::in_pos(t, s);
@@ -228,9 +230,19 @@ static int flex_host(int dir, int rev, const float* in, float* out, int* info) {
     return 0;
 }
 
-extern "C" int pf_host_flex_fft(int n, int dir, int rev, const float* in, float* out, int* info) {
+// slots: the z stage's default plan (pf::kFlexZSlots) or its alternative
+// (pf::kFlexZSlotsAlt; -2 for a length without one).
+extern "C" int pf_host_flex_fft(int n, int slots, int dir, int rev, const float* in, float* out,
+                                int* info) {
     switch (n) {
-#define PF_HOST_CASE(L) case L: return flex_host<L>(dir, rev, in, out, info);
+#define PF_HOST_CASE(L)                                                             \
+    case L:                                                                         \
+        if (slots == pf::kFlexZSlots)                                               \
+            return flex_host<L, pf::kFlexZSlots>(dir, rev, in, out, info);          \
+        if constexpr (pf::flex_zalt_ok(L))                                          \
+            if (slots == pf::kFlexZSlotsAlt)                                        \
+                return flex_host<L, pf::kFlexZSlotsAlt>(dir, rev, in, out, info);   \
+        return -2;
         PF_CT_COL_LENGTHS(PF_HOST_CASE)
 #undef PF_HOST_CASE
         default: return -1;

@@ -19,24 +19,27 @@ namespace pf {
 
 namespace {
 
-constexpr int kZEmax = 16;   // register slots per thread and transform
 constexpr int kZTmax = 32;   // threads per transform
 constexpr int kZTq = 16;     // columns per block
 
 // power-of-two threads per column: a block is 16 T threads, whole wavefronts
-// from T = 4 up
-template <int L>
-using ZF = Flex<L, kZEmax, kZTmax, false, true>;
-template <int L>
-using ZB = Flex<L, kZEmax, kZTmax, true, true>;
+// from T = 4 up.  EMAX: register slots per thread and transform (the default
+// kFlexZSlots, or the alternative kFlexZSlotsAlt plan of more threads with
+// fewer registers each, where one exists).
+template <int L, int EMAX>
+using ZF = Flex<L, EMAX, kZTmax, false, true>;
+template <int L, int EMAX>
+using ZB = Flex<L, EMAX, kZTmax, true, true>;
 
-template <int L, bool BIN>
-__global__ __launch_bounds__(kZTq * ZF<L>::T) void k_zcorr_ct(
+template <int L, bool BIN, int EMAX>
+__global__ __launch_bounds__((kZTq * ZF<L, EMAX>::T)) void k_zcorr_ct(
     float2* __restrict__ X, const float2* __restrict__ F1, const float2* __restrict__ F2,
     int plane, int64_t H, int nrot, int chunk, int zr, const float2* __restrict__ twz) {
-    constexpr int T = ZF<L>::T, E = ZF<L>::E;
-    constexpr int EI = ZF<L>::in_slots(), EO = ZF<L>::out_slots();
-    static_assert(ZB<L>::T == T && ZB<L>::E == E, "same threads and slots both ways");
+    using ZF = pf::ZF<L, EMAX>;
+    using ZB = pf::ZB<L, EMAX>;
+    constexpr int T = ZF::T, E = ZF::E;
+    constexpr int EI = ZF::in_slots(), EO = ZF::out_slots();
+    static_assert(ZB::T == T && ZB::E == E, "same threads and slots both ways");
     __shared__ float2 ex[L * kZTq];
     __shared__ float2 stw[L];
     for (int i = threadIdx.x; i < L; i += blockDim.x) stw[i] = twz[i];
@@ -50,7 +53,7 @@ __global__ __launch_bounds__(kZTq * ZF<L>::T) void k_zcorr_ct(
     float2 f1[EO], f2[EO];
 #pragma unroll
     for (int s = 0; s < EO; ++s) {
-        const int p = ZF<L>::out_pos(t, s);
+        const int p = ZF::out_pos(t, s);
         const bool ok = active && p >= 0;
         f1[s] = ok ? F1[(int64_t)p * plane + q] : c2(0.f, 0.f);
         f2[s] = ok ? F2[(int64_t)p * plane + q] : c2(0.f, 0.f);
@@ -67,11 +70,11 @@ __global__ __launch_bounds__(kZTq * ZF<L>::T) void k_zcorr_ct(
 #pragma unroll
             for (int s = 0; s < EI; ++s) {
                 // planes beyond the support were not written: they are zero
-                const int p = ZF<L>::in_pos(t, s);
+                const int p = ZF::in_pos(t, s);
                 const int sz = p <= L / 2 ? p : p - L;
                 if (active && p >= 0 && sz >= -zr && sz <= zr) v[s] = col[(int64_t)p * plane];
             }
-            flex_fft<L, kZEmax, kZTmax, false, -1, true>(v, exc, kZTq, t, stw, active);
+            flex_fft<L, EMAX, kZTmax, false, -1, true>(v, exc, kZTq, t, stw, active);
             if (BIN && g == 1) {
                 // W's spectrum feeds both A (with F1) and B (with F2)
                 float2 u[E];
@@ -80,12 +83,12 @@ __global__ __launch_bounds__(kZTq * ZF<L>::T) void k_zcorr_ct(
                     u[s] = s < EO ? cmul(cconj(v[s]), f2[s]) : c2(0.f, 0.f);
                     v[s] = s < EO ? cmul(cconj(v[s]), f1[s]) : c2(0.f, 0.f);
                 }
-                flex_fft<L, kZEmax, kZTmax, true, 1, true>(v, exc, kZTq, t, stw, active);
-                flex_fft<L, kZEmax, kZTmax, true, 1, true>(u, exc, kZTq, t, stw, active);
+                flex_fft<L, EMAX, kZTmax, true, 1, true>(v, exc, kZTq, t, stw, active);
+                flex_fft<L, EMAX, kZTmax, true, 1, true>(u, exc, kZTq, t, stw, active);
                 if (active) {
 #pragma unroll
                     for (int s = 0; s < EI; ++s) {
-                        const int p = ZB<L>::out_pos(t, s);
+                        const int p = ZB::out_pos(t, s);
                         if (p >= 0) {
                             col[(int64_t)p * plane] = v[s];
                             col[H + (int64_t)p * plane] = u[s];
@@ -96,11 +99,11 @@ __global__ __launch_bounds__(kZTq * ZF<L>::T) void k_zcorr_ct(
 #pragma unroll
                 for (int s = 0; s < E; ++s)
                     v[s] = s < EO ? cmul(cconj(v[s]), g == 2 ? f2[s] : f1[s]) : c2(0.f, 0.f);
-                flex_fft<L, kZEmax, kZTmax, true, 1, true>(v, exc, kZTq, t, stw, active);
+                flex_fft<L, EMAX, kZTmax, true, 1, true>(v, exc, kZTq, t, stw, active);
                 if (active) {
 #pragma unroll
                     for (int s = 0; s < EI; ++s) {
-                        const int p = ZB<L>::out_pos(t, s);
+                        const int p = ZB::out_pos(t, s);
                         if (p >= 0) col[(int64_t)p * plane] = v[s];
                     }
                 }
@@ -109,18 +112,18 @@ __global__ __launch_bounds__(kZTq * ZF<L>::T) void k_zcorr_ct(
     }
 }
 
-template <int L>
+template <int L, int EMAX>
 int launch_zct(const Fft3D& f, const FusedArgs& a, int zr, hipStream_t s) {
     const int plane = f.ny * f.nxh;
     const int64_t H = (int64_t)f.nz * plane;
     const dim3 grid((plane + kZTq - 1) / kZTq, (a.nrot + a.chunk_z - 1) / a.chunk_z);
-    const dim3 block(kZTq * ZF<L>::T);
+    const dim3 block(kZTq * ZF<L, EMAX>::T);
     if (a.binary_mask)
-        hipLaunchKernelGGL((k_zcorr_ct<L, true>), grid, block, 0, s, a.work, a.F1, a.F2, plane,
-                           H, a.nrot, a.chunk_z, zr, f.twz);
+        hipLaunchKernelGGL((k_zcorr_ct<L, true, EMAX>), grid, block, 0, s, a.work, a.F1, a.F2,
+                           plane, H, a.nrot, a.chunk_z, zr, f.twz);
     else
-        hipLaunchKernelGGL((k_zcorr_ct<L, false>), grid, block, 0, s, a.work, a.F1, a.F2, plane,
-                           H, a.nrot, a.chunk_z, zr, f.twz);
+        hipLaunchKernelGGL((k_zcorr_ct<L, false, EMAX>), grid, block, 0, s, a.work, a.F1, a.F2,
+                           plane, H, a.nrot, a.chunk_z, zr, f.twz);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_error("launch of k_zcorr_ct failed: %s", hipGetErrorString(e));
@@ -147,8 +150,14 @@ int launch_zcorr_ct(const Fft3D& f, const FusedArgs& a, int zr, hipStream_t s) {
         set_error("z stage: invalid batch or chunk size");
         return kErrArg;
     }
+    // schedule bit 0: the alternative register plan where the length has one
+    const bool alt = a.zsched & 1;
     switch (f.nz) {
-#define PF_ZCASE(L) case L: return launch_zct<L>(f, a, zr, s);
+#define PF_ZCASE(L)                                                          \
+    case L:                                                                  \
+        if constexpr (flex_zalt_ok(L))                                       \
+            if (alt) return launch_zct<L, kFlexZSlotsAlt>(f, a, zr, s);      \
+        return launch_zct<L, kFlexZSlots>(f, a, zr, s);
         PF_CT_COL_LENGTHS(PF_ZCASE)
 #undef PF_ZCASE
         default:

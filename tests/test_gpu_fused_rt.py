@@ -112,6 +112,32 @@ def test_fused_rt_without_binary_shortcut():
     assert_scan_parity(corr.lcc, corr.rot, ref)
 
 
+@pytest.mark.parametrize("name", ["c120_lc", "s84x96x112", "s70x72x98_cw", "s60x64x75_cw",
+                                  "s48x50x45"])
+def test_alternative_z_register_plan_vs_oracle(name):
+    """z schedule 1: the run-time planned z stage's alternative register plan
+    (8 slots per thread) where the z length has one -- 120, 84, 70, 60 and 48
+    here -- within FP32 rounding of the default plan and within the bar of
+    the oracle."""
+    w = CASES[name]
+    c = make_case(w)
+    R = rotations_for_angle(w.angle)[:8]
+    say(f"scan {name} {c.target.shape} fused_rt, z schedule 1 vs 0, 8 rotations")
+    a = scan(c, R, "fused_rt")
+    b = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused_rt", batch=8)
+    b.template = c.template
+    b.mask = c.mask
+    b.rotations = R
+    b.set_schedule(1)
+    b.scan()
+    d = float(np.abs(a.lcc - b.lcc).max())
+    ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
+                           laplace=c.laplace, nproc=NPROC)
+    err, dec = assert_scan_parity(b.lcc, b.rot, ref)
+    say(f"  |schedule 1 - schedule 0| {d:.2e}; max err {err:.2e}, decisive voxels {dec}")
+    assert d <= 1e-5
+
+
 def test_fused_rt_on_a_compiled_size_matches_oracle():
     """Forced onto a grid that also has compiled kernels (64^3): the same
     results within the tolerance."""

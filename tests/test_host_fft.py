@@ -112,23 +112,41 @@ def test_compile_time_length_passes_match_numpy(host, n):
         assert np.abs(out - ref).max() / np.abs(ref).max() < 2e-6, (n, d, radices)
 
 
+@pytest.mark.parametrize("slots", [16, 8])
 @pytest.mark.parametrize("n", [n for n in CT_LENGTHS if n >= 16])
-def test_register_fft_with_stagewise_butterflies_matches_numpy(host, n):
+def test_register_fft_with_stagewise_butterflies_matches_numpy(host, n, slots):
     """The register FFTs of the run-time planned z stage (each stage with its
-    own butterflies per thread, 16 register slots) equal numpy's transform,
-    forward and inverse, in both stage orders; the forward plan's output
-    pattern is the reversed plan's input pattern and vice versa (the z stage
-    multiplies and transforms back without an exchange).  A wrong idle-slot
-    test or a misplaced output would fail here."""
+    own butterflies per thread; the default plan of 16 register slots and the
+    alternative of 8, z schedule bit 0, where a length has one) equal numpy's
+    transform, forward and inverse, in both stage orders; the forward plan's
+    output pattern is the reversed plan's input pattern and vice versa (the z
+    stage multiplies and transforms back without an exchange).  A wrong
+    idle-slot test or a misplaced output would fail here."""
     rng = np.random.default_rng(n + 11)
     x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
     info = (ctypes.c_int * 4)()
+    probe = np.zeros(n, np.complex64)
+    if host.pf_host_flex_fft(n, slots, -1, 0, x.ctypes.data_as(FP), probe.ctypes.data_as(FP),
+                             info) == -2:
+        assert slots == 8
+        pytest.skip(f"no {slots}-slot register plan for length {n}")
     for d in (-1, 1):
         ref = np.fft.fft(x.astype(np.complex128)) if d < 0 else np.fft.ifft(x) * n
         for rev in (0, 1):
             out = np.full(n, np.nan, np.complex64)
-            assert host.pf_host_flex_fft(n, d, rev, x.ctypes.data_as(FP),
+            assert host.pf_host_flex_fft(n, slots, d, rev, x.ctypes.data_as(FP),
                                          out.ctypes.data_as(FP), info) == 0
             assert np.abs(out - ref).max() / np.abs(ref).max() < 2e-6, (n, d, rev, list(info))
     t, e, ns, same = list(info)
-    assert same == 1 and e <= 16 and t in (1, 2, 4, 8, 16, 32) and 1 <= ns <= 3
+    assert same == 1 and e <= slots and t in (1, 2, 4, 8, 16, 32) and 1 <= ns <= 3
+
+
+def test_alternative_register_plans_cover_the_cli_lengths(host):
+    """The lengths the command line's grids take most (70, 75, 84, 96, 98,
+    105, 112, 120, 126, 128) all have the alternative 8-slot plan."""
+    info = (ctypes.c_int * 4)()
+    for n in (70, 75, 84, 96, 98, 105, 112, 120, 126, 128):
+        x = np.ones(n, np.complex64)
+        out = np.zeros(n, np.complex64)
+        assert host.pf_host_flex_fft(n, 8, -1, 0, x.ctypes.data_as(FP),
+                                     out.ctypes.data_as(FP), info) == 0, n
