@@ -3,8 +3,9 @@
 Times short scans of the correlator's own inputs over candidate rotation-batch
 sizes and per-block rotation chunks of the fused kernels, then, at the best
 of those, the other schedules of the fused z stage (alternative FFT radix
-plan, alternative column tile width, both; for the run-time planned z stage
-its alternative register plan), and keeps the fastest.  Results are cached per (device, grid, mask kind, mode, precision) in a JSON
+plan, alternative column tile width, both, and the run-time planned
+pipeline's register z stage with either register plan; for the run-time
+planned pipeline its alternative register plan), and keeps the fastest.  Results are cached per (device, grid, mask kind, mode, precision) in a JSON
 file (``$POWERFIT_AMD_TUNE`` or ``~/.cache/powerfit_amd/tune.json``).  Batch
 and chunk sizes do not change the result (every rotation's LCC map is computed
 identically and merged with an order-independent max); the FFT schedule
@@ -63,7 +64,7 @@ def _time(corr, batch, chunks, nrot, repeats=2, z_schedule=0):
 
 
 def autotune(corr, batches=DEFAULT_BATCHES, chunks=DEFAULT_CHUNKS, nrot=None, use_cache=True,
-             verbose=False, schedules=(0, 1, 2, 3)):
+             verbose=False, schedules=(0, 1, 2, 3, 4, 5)):
     """Pick (batch, chunk_z, chunk_lcc, z_schedule) for ``corr`` (a ready
     HipCorrelator).
 

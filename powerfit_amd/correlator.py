@@ -228,10 +228,12 @@ class HipCorrelator(BaseCorrelator):
                    "pf_ctx_set_tuning")
 
     def set_schedule(self, z_schedule=0):
-        """Schedule of the fused z stage (0 .. 3; bit 0 the alternative radix
-        plan, bit 1 the alternative column tile width; 0 is the default).  The
-        run-time planned z stage reads bit 0 only (its alternative register
-        plan, where the z length has one)."""
+        """Schedule of the fused z stage (0 .. 5; bit 0 the alternative radix
+        plan, bit 1 the alternative column tile width, 4 and 5 the run-time
+        planned pipeline's register z stage with its default or alternative
+        register plan; 0 is the default).  The run-time planned z stage reads
+        bit 0 only (its alternative register plan, where the z length has
+        one)."""
         _lib.check(self._libref.pf_ctx_set_schedule(self._ctx, int(z_schedule)),
                    "pf_ctx_set_schedule")
         self.z_schedule = int(z_schedule)

@@ -1274,6 +1274,17 @@ int run_fused(const FusedArgs& a, int which, hipStream_t s) {
     if (which < 0 || which == 1) {
         const int ch = a.chunk_z;
         const bool alt_plan = a.zsched & 1, alt_tile = a.zsched & 2;
+        if (!a.split_z && (a.zsched & 4)) {
+            // schedules 4 and 5: the register z stage of the run-time planned
+            // pipeline (same work-buffer layout), bit 0 its register plan
+            Fft3D fz{};
+            fz.nz = N;
+            fz.ny = N;
+            fz.nx = N;
+            fz.nxh = N / 2 + 1;
+            fz.twz = tw.N;
+            return launch_zcorr_ct(fz, a, zr_read, s);
+        }
         if (a.split_z) {
             // split-precision first stage: the default schedule only
             if constexpr (zsplit_ok<N, TQ, SH::EMAXZ>()) {

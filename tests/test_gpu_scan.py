@@ -406,17 +406,18 @@ SCHEDULE_CASES = [("f48", 48), ("cpu64", 64), ("f72", 72), ("f80", 80), ("lc96",
 
 @pytest.mark.parametrize("name,n", SCHEDULE_CASES)
 def test_alternative_z_schedule_vs_oracle(name, n):
-    """Every z-stage schedule (radix plan x column tile) at every fused size:
-    oracle parity, agreement with the default to FP32 rounding, and the tile
-    width alone changes nothing."""
+    """Every z-stage schedule (radix plan x column tile, and the register z
+    stage with either register plan) at every fused size: oracle parity,
+    agreement with the default to FP32 rounding, and the tile width alone
+    changes nothing."""
     w = NEW_SIZES[n] if n in NEW_SIZES else WORKLOADS[name]
     c = make_case(w)
     R = rotations_for_angle(w.angle)[:24]
     ref = oc.parallel_scan(c.target.astype(np.float64), c.template, c.mask, R,
                            laplace=c.laplace, nproc=NPROC)
-    say(f"scan {n}^3 fused, z schedules 0-3, 24 rotations")
+    say(f"scan {n}^3 fused, z schedules 0-5, 24 rotations")
     outs = []
-    for zs in range(4):
+    for zs in range(6):
         corr = HipCorrelator(c.target.astype(np.float64), laplace=c.laplace, mode="fused",
                              batch=8)
         corr.set_schedule(zs)
@@ -427,7 +428,7 @@ def test_alternative_z_schedule_vs_oracle(name, n):
         err, dec = assert_scan_parity(corr.lcc, corr.rot, ref)
         say(f"  schedule {zs}: max err {err:.2e}, decisive voxels {dec}")
         outs.append((corr.lcc.copy(), corr.rot.copy()))
-    for zs in (1, 3):
+    for zs in (1, 3, 4, 5):
         assert np.abs(outs[0][0] - outs[zs][0]).max() < 1e-5
     for plan in (0, 1):   # schedules plan and plan + 2 differ in the tile width only
         assert np.array_equal(outs[plan][0], outs[plan + 2][0])
