@@ -204,14 +204,15 @@ int pf_ctx_set_tuning(pf_ctx *ctx, int chunk_z, int chunk_lcc);
  * selects the radix plan of its FFTs (the default, or an alternative with a
  * different element count per thread and radix sequence), bit 1 the width of
  * the (ky, kx) column tile per block (the default, or an alternative width);
- * all four are compiled for every fused size, 0 being that size's measured
- * best.  4 and 5: the z stage of the run-time planned pipeline (columns in
+ * all four are compiled for every fused size.  4 and 5: the z stage of the
+ * run-time planned pipeline (columns in
  * registers with per-stage butterfly counts), with its default (4) or
  * alternative (5) register plan; PF_MODE_FUSED_RT always runs that stage and
  * reads bit 0 only (the alternative plan, 8 slots per thread instead of 16,
- * where the z length has one).  Picked by the autotuner; the plans change
- * results by FP32 rounding of the FFTs, the tile width not at all.  No
- * counterpart in the reference. */
+ * where the z length has one).  A new context starts at its size's measured
+ * best (4 at 160^3 and 192^3, else 0); the autotuner picks per inputs.  The
+ * plans change results by FP32 rounding of the FFTs, the tile width not at
+ * all.  No counterpart in the reference. */
 #define PF_Z_SCHEDULES 6
 int pf_ctx_set_schedule(pf_ctx *ctx, int z_schedule);
 int pf_ctx_profile(pf_ctx *ctx, int enable);

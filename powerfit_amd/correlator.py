@@ -138,7 +138,10 @@ class HipCorrelator(BaseCorrelator):
         self._dev = device
         self.mode = mode
         self.batch = int(batch)
-        self.z_schedule = 0
+        # the library's default for this grid (pf_ctx_create): the register
+        # z stage at 160^3 and 192^3, else schedule 0
+        n = self._target.shape
+        self.z_schedule = 4 if n[0] == n[1] == n[2] and n[0] in (160, 192) else 0
         lib = _lib.load()
         self._libref = lib
         ctx = ctypes.c_void_p()
