@@ -274,8 +274,9 @@ def main(argv=None):
     ap.add_argument("--chunks", default=None,
                     help="rotations per block of the fused z and LCC kernels, 'cz,cl' "
                          "(with --batch; default automatic)")
-    ap.add_argument("--z-schedule", type=int, default=0,
-                    help="schedule of the fused z stage (with --batch)")
+    ap.add_argument("--z-schedule", type=int, default=None,
+                    help="schedule of the fused z stage (with --batch; default: the "
+                         "library's default for the grid)")
     ap.add_argument("--mode", default="fused", choices=["fused", "fused_rt", "batched", "plain", "auto"])
     ap.add_argument("--workload", default="search128")
     ap.add_argument("--plain-rotations", type=int, default=256,
@@ -351,7 +352,8 @@ def main(argv=None):
         if args.chunks:
             cz, cl = (int(v) for v in args.chunks.split(","))
             corr.set_tuning(cz, cl)
-        corr.set_schedule(args.z_schedule)
+        if args.z_schedule is not None:
+            corr.set_schedule(args.z_schedule)
     else:
         from powerfit_amd.autotune import autotune
         tuning = autotune(corr, use_cache=False)
