@@ -210,7 +210,8 @@ int pf_ctx_set_tuning(pf_ctx *ctx, int chunk_z, int chunk_lcc);
  * alternative (5) register plan; PF_MODE_FUSED_RT always runs that stage and
  * reads bit 0 only (the alternative plan, 8 slots per thread instead of 16,
  * where the z length has one).  A new context starts at its size's measured
- * best (4 at 160^3 and 192^3, else 0); the autotuner picks per inputs.  The
+ * best (4 at 64^3, 72^3, 96^3, 160^3 and 192^3, else 0); the autotuner
+ * picks per inputs.  The
  * plans change results by FP32 rounding of the FFTs, the tile width not at
  * all.  No counterpart in the reference. */
 #define PF_Z_SCHEDULES 6

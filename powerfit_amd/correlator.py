@@ -139,9 +139,9 @@ class HipCorrelator(BaseCorrelator):
         self.mode = mode
         self.batch = int(batch)
         # the library's default for this grid (pf_ctx_create): the register
-        # z stage at 160^3 and 192^3, else schedule 0
+        # z stage at 64^3, 72^3, 96^3, 160^3 and 192^3, else schedule 0
         n = self._target.shape
-        self.z_schedule = 4 if n[0] == n[1] == n[2] and n[0] in (160, 192) else 0
+        self.z_schedule = 4 if n[0] == n[1] == n[2] and n[0] in (64, 72, 96, 160, 192) else 0
         lib = _lib.load()
         self._libref = lib
         ctx = ctypes.c_void_p()

@@ -489,9 +489,13 @@ int pf_ctx_create(int device, int nz, int ny, int nx, uint32_t flags, pf_ctx** o
         if (c->num_cu <= 0) c->num_cu = 256;
     }
     if (!s) s = fft3d_init(&c->fft, nz, ny, nx);
-    // default z schedule: the size's measured best, the register z stage at
-    // 160^3 and 192^3 (K2 21.0 -> 17.7 and 33.0 -> 28.2 us per rotation)
-    if (nz == ny && ny == nx && (nz == 160 || nz == 192)) c->zsched = 4;
+    // default z schedule: the size's measured best at batch 256 with automatic
+    // chunks, the register z stage at 64^3, 72^3, 96^3, 160^3 and 192^3 (K2 per
+    // rotation 1.11 -> 1.03, 1.87 -> 1.74, 3.68 -> 3.50, 21.0 -> 17.7 and
+    // 33.0 -> 28.2 us)
+    if (nz == ny && ny == nx &&
+        (nz == 64 || nz == 72 || nz == 96 || nz == 160 || nz == 192))
+        c->zsched = 4;
     if (!s && plane_rt_supported(nz, ny, nx)) s = prepare_planes_rt();
     if (!s && (flags & PF_FLAG_SPLIT_DFT)) s = fft3d_set_split(&c->fft, true);
     if (!s && hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
